@@ -1,0 +1,376 @@
+// Interval-set algebra on int32 CSR sets for gfx950.
+//
+// Replaces the reference's one-thread-per-row serial walks
+// (subsetix_cupy/kernels.py:11-236 count_segments / write_segments, and the
+// sort + merge rebuild kernels.py:292-381) with a lane-parallel merge:
+//
+//   * Each output row is handled by a G-lane group (G = 64: one wavefront per
+//     row; G < 64: several short rows per wavefront).  The group walks the
+//     row's merged order (A and B merged by key) G positions at a time; lane l
+//     finds the element at merged position l by a co-rank (merge-path)
+//     binary search.
+//   * union: a group max-scan of interval ends gives, for every element, the
+//     largest end before it; an element starts an output run when its begin
+//     is above that value (the reference's `seg_start <= active_end` merge
+//     rule).  This handles operands whose intervals overlap (widened or
+//     coarsened inputs), so the same code is the "rebuild" merge.
+//   * intersection / difference: every output piece starts either at an A
+//     element or at a B element; whether it does, and where the piece ends,
+//     follows from the element's neighbours in merged order.  Exact for
+//     operands whose rows are sorted and non-overlapping (the reference's
+//     documented input contract, expressions.py:420-425).
+//   * count, scan and write run in ONE kernel: a block counts its tile of
+//     rows, takes its global output offset from a decoupled look-back over
+//     earlier tiles, then writes (the second walk reads from L1/L2).
+#include "stx_common.h"
+#include "stx_fused.h"
+
+#include <limits.h>
+
+using namespace stx;
+
+namespace {
+
+struct Operand {
+    const int32_t* beg;
+    const int32_t* end;
+    const int32_t* off;  // nullptr: operand is empty
+    const int32_t* map;  // output row -> source row (-1 = empty); nullptr = identity
+    XForm xf;
+};
+
+__device__ __forceinline__ void row_extent(const Operand& X, int64_t u, int32_t& x0, int32_t& nx) {
+    x0 = 0;
+    nx = 0;
+    if (X.off == nullptr) return;
+    int64_t src = X.map ? (int64_t)X.map[u] : u;
+    if (src < 0) return;
+    x0 = X.off[src];
+    nx = X.off[src + 1] - x0;
+}
+
+__device__ __forceinline__ int32_t keyb(const Operand& X, int32_t k) {
+    int32_t b = X.beg[k], e = 0;
+    XForm xf = X.xf;
+    if (xf.mode == XF_WIDEN) return b - xf.p;
+    if (xf.mode == XF_COARSEN) return floordiv(b, xf.p);
+    if (xf.mode == XF_CLIP) return b < 0 ? 0 : b;
+    if (xf.mode == XF_SCALE) return b * xf.p;
+    (void)e;
+    return b;
+}
+
+__device__ __forceinline__ int32_t keye(const Operand& X, int32_t k) {
+    int32_t e = X.end[k];
+    XForm xf = X.xf;
+    if (xf.mode == XF_WIDEN) return e + xf.p;
+    if (xf.mode == XF_COARSEN) return floordiv(e + xf.p - 1, xf.p);
+    if (xf.mode == XF_CLIP) return e > xf.p ? xf.p : e;
+    if (xf.mode == XF_SCALE) return e * xf.p;
+    return e;
+}
+
+// true if A element ia precedes B element jb in the merged order of `OP`
+template <int OP>
+__device__ __forceinline__ bool a_first(const Operand& A, int32_t ia, const Operand& B, int32_t jb) {
+    if (OP == 2) return keyb(A, ia) < keye(B, jb);  // difference: A by begin, B by end, B first on ties
+    return keyb(A, ia) <= keyb(B, jb);             // union / intersection: both by begin, A first
+}
+
+// Processes one output row with a G-lane group.  Returns the number of
+// output intervals (uniform across the group).  With WRITE, writes them at
+// ob/oe[base ...].
+template <int G, int OP, bool WRITE>
+__device__ __forceinline__ int32_t setop_row(const Operand& A, int32_t a0, int32_t na, const Operand& B, int32_t b0, int32_t nb,
+                             int32_t* __restrict__ ob, int32_t* __restrict__ oe, int64_t base) {
+    const int gl = lane_id() & (G - 1);
+    const int32_t n = na + nb;
+    int32_t count = 0;
+    int32_t carry_pm = INT_MIN;
+    for (int32_t c = 0; c < n; c += G) {
+        const int32_t l = c + gl;
+        const bool valid = l < n;
+        bool emit = false;
+        int32_t pb = 0, pe = 0;
+        int32_t eb = 0, ee = INT_MIN;
+        bool alive = false;
+        if (valid) {
+            int32_t lo = l - nb > 0 ? l - nb : 0;
+            int32_t hi = l < na ? l : na;
+            while (lo < hi) {
+                int32_t mid = (lo + hi) >> 1;
+                if (a_first<OP>(A, a0 + mid, B, b0 + (l - 1 - mid))) lo = mid + 1;
+                else hi = mid;
+            }
+            const int32_t i = lo, j = l - lo;
+            const bool isA = (j >= nb) || (i < na && a_first<OP>(A, a0 + i, B, b0 + j));
+            if (OP == 0) {
+                if (isA) { eb = keyb(A, a0 + i); ee = keye(A, a0 + i); }
+                else { eb = keyb(B, b0 + j); ee = keye(B, b0 + j); }
+                const bool clip_a = A.xf.mode == XF_CLIP, clip_b = B.xf.mode == XF_CLIP;
+                alive = !((isA ? clip_a : clip_b) && eb >= ee);
+                if (!alive) ee = INT_MIN;
+            } else if (OP == 1) {
+                if (isA) {
+                    const int32_t ab = keyb(A, a0 + i), ae = keye(A, a0 + i);
+                    int32_t ce = INT_MIN;
+                    if (j >= 1 && keye(B, b0 + j - 1) > ab) ce = keye(B, b0 + j - 1);
+                    else if (j < nb && keyb(B, b0 + j) == ab) ce = keye(B, b0 + j);
+                    if (ce != INT_MIN) { emit = true; pb = ab; pe = ae < ce ? ae : ce; }
+                } else {
+                    const int32_t bb = keyb(B, b0 + j), be = keye(B, b0 + j);
+                    if (i >= 1) {
+                        const int32_t pab = keyb(A, a0 + i - 1), pae = keye(A, a0 + i - 1);
+                        if (pab < bb && bb < pae) { emit = true; pb = bb; pe = be < pae ? be : pae; }
+                    }
+                }
+            } else {
+                if (isA) {
+                    const int32_t ab = keyb(A, a0 + i), ae = keye(A, a0 + i);
+                    const bool covered = j < nb && keyb(B, b0 + j) <= ab;
+                    if (!covered) {
+                        emit = true;
+                        pb = ab;
+                        pe = ae;
+                        if (j < nb) { const int32_t nbg = keyb(B, b0 + j); pe = pe < nbg ? pe : nbg; }
+                    }
+                } else {
+                    const int32_t be = keye(B, b0 + j);
+                    if (i >= 1) {
+                        const int32_t pae = keye(A, a0 + i - 1);
+                        const bool open = (j + 1 >= nb) || keyb(B, b0 + j + 1) > be;
+                        if (be < pae && open) {
+                            emit = true;
+                            pb = be;
+                            pe = pae;
+                            if (j + 1 < nb) { const int32_t nbg = keyb(B, b0 + j + 1); pe = pe < nbg ? pe : nbg; }
+                        }
+                    }
+                }
+            }
+        }
+        if (OP == 0) {
+            int32_t pm_incl = group_scan_max<G>(ee);
+            pm_incl = pm_incl > carry_pm ? pm_incl : carry_pm;
+            int32_t pm_excl = __shfl_up(pm_incl, 1, G);
+            if (gl == 0) pm_excl = carry_pm;
+            const bool start = valid && alive && eb > pm_excl;
+            const uint64_t m = group_ballot<G>(start);
+            if (WRITE && start) {
+                const int32_t k = count + rank_below(m, gl);
+                ob[base + k] = eb;
+                if (k >= 1) oe[base + k - 1] = pm_excl;
+            }
+            count += __popcll(m);
+            carry_pm = __shfl(pm_incl, G - 1, G);
+        } else {
+            const uint64_t m = group_ballot<G>(emit);
+            if (WRITE && emit) {
+                const int32_t k = count + rank_below(m, gl);
+                ob[base + k] = pb;
+                oe[base + k] = pe;
+            }
+            count += __popcll(m);
+        }
+    }
+    if (OP == 0 && WRITE && count >= 1 && gl == 0) oe[base + count - 1] = carry_pm;
+    return count;
+}
+
+template <int OP>
+struct SetopRow {
+    Operand A, B;
+    template <int G, bool WRITE>
+    __device__ int32_t run(int64_t u, int32_t* ob, int32_t* oe, int64_t base) const {
+        int32_t a0, na, b0, nb;
+        row_extent(A, u, a0, na);
+        row_extent(B, u, b0, nb);
+        return setop_row<G, OP, WRITE>(A, a0, na, B, b0, nb, ob, oe, base);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Generic exclusive scan of int32 counts (single pass, decoupled look-back).
+// out[k] = sum(in[0..k)), out[n] = total; *d_total = total.
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kBlock * kScanItems;
+
+__global__ __launch_bounds__(kBlock) void scan_i32_kernel(const int32_t* __restrict__ in, const int32_t* d_n, int64_t n_bound,
+                                                          uint32_t ntiles, int32_t* __restrict__ out,
+                                                          int32_t* __restrict__ d_total, uint32_t* status, void* scratch,
+                                                          int64_t spin_limit) {
+    __shared__ int32_t s_scan[17];
+    __shared__ uint32_t s_slot;
+    __shared__ uint32_t s_excl;
+    LookbackScratch lb = lookback_scratch(scratch, ntiles);
+    const uint32_t tile = lookback_ticket(lb, &s_slot);
+    const int64_t n = d_n ? (int64_t)*d_n : n_bound;
+    const int64_t t0 = (int64_t)tile * kScanTile + (int64_t)threadIdx.x * kScanItems;
+    int32_t vals[kScanItems];
+    int32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        int64_t idx = t0 + k;
+        vals[k] = idx < n ? in[idx] : 0;
+        sum += vals[k];
+    }
+    int32_t total = 0;
+    int32_t ex = block_exclusive_scan<int32_t>(sum, s_scan, &total);
+    if (wave_id() == 0) {
+        uint32_t e = lookback_exclusive(lb, tile, (uint32_t)total, status, spin_limit);
+        if (lane_id() == 0) s_excl = e;
+    }
+    __syncthreads();
+    int32_t run = (int32_t)s_excl + ex;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        int64_t idx = t0 + k;
+        if (idx <= n) out[idx] = run;
+        if (idx == n && d_total) *d_total = run;
+        run += vals[k];
+    }
+    if (threadIdx.x == kBlock - 1 && t0 + kScanItems == n) {
+        out[n] = run;
+        if (d_total) *d_total = run;
+    }
+    lookback_finish(lb, ntiles, &s_slot);
+}
+
+// ---------------------------------------------------------------------------
+// Row-id alignment: the sorted union of two strictly increasing row lists
+// (each optionally shifted) and, per union row, the source row in each list
+// (-1 when absent).  Replaces _align_interval_sets / _reindex_interval_set
+// (expressions.py:58-141): the intervals themselves are never copied, the
+// set-op kernel reads them through the maps.
+__device__ __forceinline__ int64_t lower_bound_shift(const int32_t* a, int64_t n, int32_t shift, int32_t v) {
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if (a[mid] + shift < v) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void rows_newflag_kernel(const int32_t* __restrict__ ra, int64_t na, int32_t sa, const int32_t* __restrict__ rb,
+                                    int64_t nb, int32_t sb, int32_t* __restrict__ flag) {
+    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nb) return;
+    int32_t v = rb[j] + sb;
+    int64_t p = lower_bound_shift(ra, na, sa, v);
+    flag[j] = (p < na && ra[p] + sa == v) ? 0 : 1;
+}
+
+__global__ void rows_scatter_kernel(const int32_t* __restrict__ ra, int64_t na, int32_t sa, const int32_t* __restrict__ rb,
+                                    int64_t nb, int32_t sb, const int32_t* __restrict__ flag,
+                                    const int32_t* __restrict__ newpos, int32_t* __restrict__ rows_u,
+                                    int32_t* __restrict__ map_a, int32_t* __restrict__ map_b) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < na) {
+        int32_t v = ra[t] + sa;
+        int64_t jl = lower_bound_shift(rb, nb, sb, v);
+        int64_t u = t + newpos[jl];
+        rows_u[u] = v;
+        map_a[u] = (int32_t)t;
+        map_b[u] = (jl < nb && rb[jl] + sb == v) ? (int32_t)jl : -1;
+    } else if (t < na + nb) {
+        int64_t j = t - na;
+        if (!flag[j]) return;
+        int32_t v = rb[j] + sb;
+        int64_t u = newpos[j] + lower_bound_shift(ra, na, sa, v);
+        rows_u[u] = v;
+        map_a[u] = -1;
+        map_b[u] = (int32_t)j;
+    }
+}
+
+__global__ void fill_i32_kernel(int32_t* p, int64_t n, int32_t v, int32_t step) {
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = v + (int32_t)k * step;
+}
+
+__global__ void count_plus_kernel(int32_t* dst, const int32_t* src, int32_t add) { *dst = *src + add; }
+
+__global__ void add_i32_kernel(int32_t* p, int64_t n, int32_t v) {
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) p[k] += v;
+}
+
+}  // namespace
+
+// ===========================================================================
+// Host side (C-ABI).
+#include "stx_host.h"
+
+template <int G>
+static int32_t launch_setop_g(int32_t op, const Operand& A, const Operand& B, const int32_t* d_nrows, int64_t nrows_bound,
+                              int32_t* out_off, int32_t* ob, int32_t* oe, int64_t capacity, int32_t* d_total,
+                              uint32_t* status, void* scratch, int64_t scratch_bytes, hipStream_t stream) {
+    FusedLaunch L{d_nrows, nrows_bound, out_off, ob, oe, capacity, d_total, status, scratch, scratch_bytes, stream};
+    if (op == 0) return launch_fused_rows<G>(SetopRow<0>{A, B}, L, "setop_union");
+    if (op == 1) return launch_fused_rows<G>(SetopRow<1>{A, B}, L, "setop_intersection");
+    return launch_fused_rows<G>(SetopRow<2>{A, B}, L, "setop_difference");
+}
+
+extern "C" {
+
+int32_t stx_scan_i32(const int32_t* in, const int32_t* d_n, int64_t n_bound, int32_t* out, int32_t* d_total,
+                     uint32_t* status, void* scratch, int64_t scratch_bytes, hipStream_t stream) {
+    if (n_bound < 0) return STX_EINVAL;
+    uint32_t ntiles = (uint32_t)((n_bound + 1 + kScanTile - 1) / kScanTile);
+    if (ntiles == 0) ntiles = 1;
+    if (lookback_bytes(ntiles) > scratch_bytes) return stx_fail(STX_EINVAL, "scan: scratch too small");
+    hipLaunchKernelGGL(scan_i32_kernel, dim3(ntiles), dim3(kBlock), 0, stream, in, d_n, n_bound, ntiles, out, d_total,
+                       status, scratch, stx_spin_limit());
+    return stx_launch_status("scan_i32_kernel");
+}
+
+int64_t stx_scan_scratch_bytes(int64_t n_bound) {
+    int64_t ntiles = (n_bound + 1 + kScanTile - 1) / kScanTile;
+    return lookback_bytes(ntiles < 1 ? 1 : ntiles);
+}
+
+int64_t stx_setop_scratch_bytes(int64_t nrows_bound, int32_t group) {
+    return fused_rows_scratch_bytes(nrows_bound, group);
+}
+
+int32_t stx_setop(int32_t op, const int32_t* a_beg, const int32_t* a_end, const int32_t* a_off, const int32_t* a_map,
+                  int32_t a_xf_mode, int32_t a_xf_param, const int32_t* b_beg, const int32_t* b_end, const int32_t* b_off,
+                  const int32_t* b_map, int32_t b_xf_mode, int32_t b_xf_param, const int32_t* d_nrows, int64_t nrows_bound,
+                  int32_t group, int32_t* out_off, int32_t* out_beg, int32_t* out_end, int64_t capacity,
+                  int32_t* d_total, uint32_t* status, void* scratch, int64_t scratch_bytes, hipStream_t stream) {
+    if (op < 0 || op > 2 || nrows_bound < 0 || a_off == nullptr) return stx_fail(STX_EINVAL, "setop: bad arguments");
+    if (op != 0 && b_off == nullptr) return stx_fail(STX_EINVAL, "setop: intersection/difference need two operands");
+    Operand A{a_beg, a_end, a_off, a_map, XForm{a_xf_mode, a_xf_param}};
+    Operand B{b_beg, b_end, b_off, b_map, XForm{b_xf_mode, b_xf_param}};
+    if ((a_xf_mode == XF_COARSEN && a_xf_param < 1) || (b_xf_mode == XF_COARSEN && b_xf_param < 1))
+        return stx_fail(STX_EINVAL, "setop: coarsening ratio must be >= 1");
+    if (group >= 64)
+        return launch_setop_g<64>(op, A, B, d_nrows, nrows_bound, out_off, out_beg, out_end, capacity, d_total, status,
+                                  scratch, scratch_bytes, stream);
+    if (group >= 16)
+        return launch_setop_g<16>(op, A, B, d_nrows, nrows_bound, out_off, out_beg, out_end, capacity, d_total, status,
+                                  scratch, scratch_bytes, stream);
+    return launch_setop_g<4>(op, A, B, d_nrows, nrows_bound, out_off, out_beg, out_end, capacity, d_total, status,
+                             scratch, scratch_bytes, stream);
+}
+
+int32_t stx_rows_union(const int32_t* rows_a, int64_t na, int32_t shift_a, const int32_t* rows_b, int64_t nb,
+                       int32_t shift_b, int32_t* rows_u, int32_t* map_a, int32_t* map_b, int32_t* d_nu, int32_t* tmp,
+                       uint32_t* status, void* scratch, int64_t scratch_bytes, hipStream_t stream) {
+    // tmp: 2*(nb+1) int32 (flags, positions)
+    if (na < 0 || nb < 0) return stx_fail(STX_EINVAL, "rows_union: bad sizes");
+    int32_t* flag = tmp;
+    int32_t* pos = tmp + (nb + 1);
+    if (nb > 0)
+        hipLaunchKernelGGL(rows_newflag_kernel, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, stream, rows_a, na, shift_a,
+                           rows_b, nb, shift_b, flag);
+    int32_t rc = stx_scan_i32(flag, nullptr, nb, pos, nullptr, status, scratch, scratch_bytes, stream);
+    if (rc) return rc;
+    if (na + nb > 0)
+        hipLaunchKernelGGL(rows_scatter_kernel, dim3((unsigned)((na + nb + 255) / 256)), dim3(256), 0, stream, rows_a, na,
+                           shift_a, rows_b, nb, shift_b, flag, pos, rows_u, map_a, map_b);
+    hipLaunchKernelGGL(count_plus_kernel, dim3(1), dim3(1), 0, stream, d_nu, pos + nb, (int32_t)na);
+    return stx_launch_status("rows_union");
+}
+
+}  // extern "C"

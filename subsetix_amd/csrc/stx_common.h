@@ -1,0 +1,248 @@
+// Shared device helpers for the gfx950 interval-set kernels.
+//
+// Wavefront size is 64 on CDNA4; every helper here is written for 64-lane
+// waves and for power-of-two "lane groups" of G <= 64 lanes inside a wave
+// (several short rows per wave).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define STX_WAVE 64
+
+namespace stx {
+
+// ---- status codes (mirrored in include/subsetix_hip.h) --------------------
+enum : int32_t {
+    STX_OK = 0,
+    STX_EINVAL = -1,      // bad argument
+    STX_ECAPACITY = -2,   // output capacity exceeded (device flag)
+    STX_EALIGN = -3,      // multilevel alignment violated
+    STX_EHIP = -4,        // HIP runtime error
+    STX_ESPIN = -5,       // look-back wait exceeded its bound
+};
+
+// Device-side error word bits (written with atomicOr into a status word).
+enum : uint32_t {
+    STX_FLAG_CAPACITY = 1u,
+    STX_FLAG_SPIN = 2u,
+    STX_FLAG_ALIGN = 4u,
+    STX_FLAG_ROWS = 8u,
+};
+
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (STX_WAVE - 1)); }
+__device__ __forceinline__ int wave_id() { return (int)(threadIdx.x / STX_WAVE); }
+
+// Ballot restricted to the G-lane group of the calling lane, shifted down so
+// that bit k is group lane k.
+template <int G>
+__device__ __forceinline__ uint64_t group_ballot(bool pred) {
+    uint64_t m = __ballot(pred);
+    if (G == 64) return m;
+    const int grp = lane_id() / G;
+    return (m >> (grp * G)) & ((1ull << G) - 1ull);
+}
+
+// Number of set bits of `mask` strictly below group lane `gl`.
+__device__ __forceinline__ int rank_below(uint64_t mask, int gl) {
+    return __popcll(gl ? (mask & ((~0ull) >> (64 - gl))) : 0ull);
+}
+
+// Inclusive max-scan over a G-lane group.
+template <int G>
+__device__ __forceinline__ int group_scan_max(int v) {
+#pragma unroll
+    for (int d = 1; d < G; d <<= 1) {
+        int o = __shfl_up(v, d, G);
+        if ((lane_id() & (G - 1)) >= d) v = v > o ? v : o;
+    }
+    return v;
+}
+
+// Inclusive sum-scan over a G-lane group.
+template <int G, typename T>
+__device__ __forceinline__ T group_scan_sum(T v) {
+#pragma unroll
+    for (int d = 1; d < G; d <<= 1) {
+        T o = __shfl_up(v, d, G);
+        if ((lane_id() & (G - 1)) >= d) v += o;
+    }
+    return v;
+}
+
+template <int G>
+__device__ __forceinline__ int group_bcast(int v, int src_gl) {
+    return __shfl(v, src_gl, G);
+}
+
+__device__ __forceinline__ int32_t floordiv(int32_t a, int32_t b) {
+    int32_t q = a / b;
+    return (a % b != 0 && ((a < 0) != (b < 0))) ? q - 1 : q;
+}
+
+// ---- element transforms ----------------------------------------------------
+// Monotone maps applied to an operand's intervals on load; they keep each
+// row's intervals sorted by begin.  A transformed interval with b >= e is
+// "dead" (only the clip transform produces those).
+enum : int32_t { XF_NONE = 0, XF_WIDEN = 1, XF_COARSEN = 2, XF_CLIP = 3, XF_SCALE = 4 };
+
+struct XForm {
+    int32_t mode;
+    int32_t p;
+};
+
+__device__ __forceinline__ void xform_apply(const XForm& xf, int32_t& b, int32_t& e) {
+    if (xf.mode == XF_WIDEN) {
+        b -= xf.p;
+        e += xf.p;
+    } else if (xf.mode == XF_COARSEN) {
+        b = floordiv(b, xf.p);
+        e = floordiv(e + xf.p - 1, xf.p);
+    } else if (xf.mode == XF_CLIP) {
+        b = b < 0 ? 0 : b;
+        e = e > xf.p ? xf.p : e;
+    } else if (xf.mode == XF_SCALE) {
+        b *= xf.p;
+        e *= xf.p;
+    }
+}
+
+// ---- decoupled look-back ---------------------------------------------------
+// Scratch layout shared by every single-pass kernel (caller-provided, zeroed
+// once at allocation, left zeroed by the last block of every launch):
+//   [0] ticket counter   [1] done counter   [2..] per-tile state words.
+// A state word packs (flag << 32) | value with flag 1 = aggregate, 2 = prefix.
+struct LookbackScratch {
+    uint32_t* ticket;
+    uint32_t* done;
+    unsigned long long* state;
+    int64_t max_tiles;
+};
+
+__device__ __forceinline__ LookbackScratch lookback_scratch(void* base, int64_t max_tiles) {
+    LookbackScratch s;
+    s.ticket = (uint32_t*)base;
+    s.done = (uint32_t*)base + 1;
+    s.state = (unsigned long long*)((char*)base + 64);
+    s.max_tiles = max_tiles;
+    return s;
+}
+
+// A state word carries its own payload, so no other memory has to be ordered
+// with it: relaxed device-scope atomics are enough.  (Acquire/release at
+// device scope would write back / invalidate the whole L2 of the XCD on every
+// poll, because the L2s of the XCDs are not coherent with each other.)
+__device__ __forceinline__ void state_store(unsigned long long* p, uint32_t flag, uint32_t value) {
+    unsigned long long w = ((unsigned long long)flag << 32) | (unsigned long long)value;
+    __hip_atomic_store(p, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ unsigned long long state_load(const unsigned long long* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Called by all 64 lanes of ONE wave of the block.  Publishes `agg` for tile
+// `tile` and returns the exclusive prefix of all earlier tiles.  The wait is
+// bounded: after `spin_limit` polls the block records STX_FLAG_SPIN in
+// `status` and proceeds with what it has (its result is then invalid, but
+// every wave still exits).
+__device__ __forceinline__ uint32_t lookback_exclusive(LookbackScratch s, uint32_t tile, uint32_t agg,
+                                                       uint32_t* status, int64_t spin_limit) {
+    const int lane = lane_id();
+    if (tile == 0) {
+        if (lane == 0) state_store(&s.state[0], 2u, agg);
+        return 0u;
+    }
+    if (lane == 0) state_store(&s.state[tile], 1u, agg);
+    uint32_t excl = 0;
+    int64_t pos = (int64_t)tile - 1;  // nearest predecessor not yet folded in
+    int64_t spins = 0;
+    while (pos >= 0) {
+        int64_t p = pos - lane;
+        unsigned long long w = p >= 0 ? state_load(&s.state[p]) : (2ull << 32);
+        uint32_t flag = (uint32_t)(w >> 32);
+        uint32_t val = (uint32_t)w;
+        uint64_t not_ready = __ballot(flag == 0u);
+        uint64_t is_prefix = __ballot(flag == 2u);
+        // lanes up to (and including) the first prefix lane are usable only if
+        // none of them is still unpublished
+        int first_prefix = is_prefix ? __ffsll((long long)is_prefix) - 1 : 64;
+        uint64_t upto = first_prefix >= 63 ? ~0ull : ((1ull << (first_prefix + 1)) - 1ull);
+        if (not_ready & upto) {
+            if (++spins > spin_limit) {
+                if (lane == 0) atomicOr(status, STX_FLAG_SPIN);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+            continue;
+        }
+        uint32_t contrib = (lane <= first_prefix && p >= 0) ? val : 0u;
+        // wave sum
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) contrib += __shfl_xor(contrib, d, 64);
+        excl += contrib;
+        if (first_prefix < 64) break;
+        pos -= 64;
+    }
+    if (lane == 0) state_store(&s.state[tile], 2u, excl + agg);
+    return excl;
+}
+
+// Dynamic tile id: tiles are numbered in the order blocks start, so every
+// tile a block waits on belongs to a block that is already running.
+__device__ __forceinline__ uint32_t lookback_ticket(LookbackScratch s, uint32_t* lds_slot) {
+    __syncthreads();
+    if (threadIdx.x == 0) *lds_slot = atomicAdd(s.ticket, 1u);
+    __syncthreads();
+    return *lds_slot;
+}
+
+// Called by ALL threads at the very end of a block: the last block to finish
+// clears the scratch (in parallel) for the next launch.
+__device__ __forceinline__ void lookback_finish(LookbackScratch s, uint32_t ntiles, uint32_t* lds_slot) {
+    // Every state read of this block has returned before the increment (the
+    // values were consumed), so the block that counts last may clear the
+    // words; the next launch sees the zeros at the kernel boundary.
+    __syncthreads();
+    if (threadIdx.x == 0) *lds_slot = atomicAdd(s.done, 1u) == ntiles - 1u ? 1u : 0u;
+    __syncthreads();
+    if (*lds_slot) {
+        for (uint32_t t = threadIdx.x; t < ntiles; t += blockDim.x)
+            __hip_atomic_store(&s.state[t], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            __hip_atomic_store(s.ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(s.done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+// Bytes of look-back scratch needed for `ntiles` tiles.
+__host__ __device__ inline int64_t lookback_bytes(int64_t ntiles) { return 64 + 8 * ntiles; }
+
+// Block-wide exclusive scan of one value per thread (blockDim.x <= 1024).
+// Returns the exclusive prefix; *total receives the block sum.
+template <typename T>
+__device__ __forceinline__ T block_exclusive_scan(T v, T* lds /* >= 17 */, T* total) {
+    const int lane = lane_id(), wave = wave_id();
+    const int nwaves = (int)(blockDim.x + STX_WAVE - 1) / STX_WAVE;
+    T incl = group_scan_sum<64, T>(v);
+    __syncthreads();
+    if (lane == 63) lds[wave] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        T acc = 0;
+        for (int w = 0; w < nwaves; ++w) {
+            T t = lds[w];
+            lds[w] = acc;
+            acc += t;
+        }
+        lds[16] = acc;
+    }
+    __syncthreads();
+    T res = lds[wave] + incl - v;
+    *total = lds[16];
+    return res;
+}
+
+}  // namespace stx

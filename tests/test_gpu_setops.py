@@ -1,0 +1,140 @@
+"""GPU parity: set algebra (fused union / intersection / difference kernel)
+against the oracle and the reference's golden vectors, bit-exact."""
+
+import numpy as np
+import pytest
+
+from oracle import oracle as ox
+from tests.golden_util import assert_set_equal, cell_model, eval_spec, oset, random_canonical
+from tests.gpu_util import hostset_to_oset, to_device, to_host
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def sx():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    import subsetix_amd
+
+    return subsetix_amd
+
+
+def _ops(sx):
+    def wrap(fn):
+        return lambda a, b: sx.evaluate(fn(sx.make_input(a), sx.make_input(b)))
+
+    return {
+        "union": wrap(sx.make_union),
+        "intersection": wrap(sx.make_intersection),
+        "difference": wrap(sx.make_difference),
+        "symmetric_difference": wrap(sx.make_symmetric_difference),
+        "complement": wrap(sx.make_complement),
+    }
+
+
+def test_golden_expressions(sx, golden):
+    surf = {k: to_device(oset(v)) for k, v in golden["surfaces"].items()}
+    ops = _ops(sx)
+    for case in golden["expressions"]:
+        print("case", case["id"], flush=True)
+        got = eval_spec(case["spec"], surf, ops)
+        assert_set_equal(to_host(got), oset(case["expect"]))
+
+
+def test_golden_cell_model(sx, golden):
+    surf_h = {k: oset(v) for k, v in golden["surfaces"].items()}
+    surf = {k: to_device(v) for k, v in surf_h.items()}
+    cells = {k: ox.cells(v) for k, v in surf_h.items()}
+    ops = _ops(sx)
+    for group in ("dense", "sparse"):
+        for spec in golden["cell_model_specs"][group]:
+            got = eval_spec(spec, surf, ops)
+            assert ox.cells(to_host(got)) == cell_model(spec, cells), spec
+
+
+@pytest.mark.parametrize("group", [None, 4, 16, 64])
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_random_vs_oracle(sx, seed, group):
+    from subsetix_amd import _core
+
+    rng = np.random.default_rng(seed)
+    _core.set_group_override(group)
+    try:
+        for rows_a, rows_b, touching in (
+            (None, None, False),
+            (np.arange(0, 40, 2), np.arange(0, 40, 3), False),
+            (np.arange(-5, 30), np.arange(10, 12), True),
+        ):
+            na = 40 if rows_a is None else rows_a.size
+            nb = 40 if rows_b is None else rows_b.size
+            a = random_canonical(rng, na, 400, mean_n=int(rng.integers(1, 80)), rows=rows_a, touching=touching)
+            b = random_canonical(rng, nb, 400, mean_n=int(rng.integers(1, 80)), rows=rows_b, touching=touching)
+            da, db = to_device(a), to_device(b)
+            for op in ("union", "intersection", "difference"):
+                print("seed", seed, "group", group, "op", op, "rows", na, nb, flush=True)
+                got = to_host(_ops(sx)[op](da, db))
+                assert_set_equal(got, ox.setop(op, a, b))
+    finally:
+        _core.set_group_override(None)
+
+
+def test_empty_and_degenerate(sx):
+    dev_empty = to_device(ox.make([0, 0, 0], [], []))
+    a = to_device(ox.make([0, 2, 3], [0, 6, 1], [4, 9, 5]))
+    ops = _ops(sx)
+    for op in ("union", "intersection", "difference"):
+        got = to_host(ops[op](dev_empty, dev_empty))
+        assert got.n == 0 and list(got.off) == [0, 0, 0]
+        assert_set_equal(to_host(ops[op](a, dev_empty)), ox.setop(op, to_host(a), to_host(dev_empty)))
+        assert_set_equal(to_host(ops[op](dev_empty, a)), ox.setop(op, to_host(dev_empty), to_host(a)))
+    zero_rows = to_device(ox.make([0], [], []))
+    got = to_host(ops["union"](zero_rows, zero_rows))
+    assert got.nrows == 0 and list(got.off) == [0]
+
+
+def test_setalgebra_config_full_size(sx):
+    """The set-algebra microbenchmark inputs (4096 rows x 65536, Poisson(32)
+    intervals per row) at full size, bit-exact."""
+    from subsetix_amd import workloads
+
+    ha, hb = workloads.setalgebra_pair()
+    a, b = hostset_to_oset(ha), hostset_to_oset(hb)
+    da, db = to_device(a), to_device(b)
+    for op in ("union", "intersection", "difference"):
+        print("config2", op, flush=True)
+        assert_set_equal(to_host(_ops(sx)[op](da, db)), ox.setop(op, a, b))
+
+
+def test_workspace_reuses_output_buffers(sx, golden):
+    """test_expressions.py:270-284: re-evaluating with a workspace reuses the
+    begin buffer."""
+    a = to_device(oset(golden["surfaces"]["A"]))
+    b = to_device(oset(golden["surfaces"]["B"]))
+    ws = sx.CuPyWorkspace()
+    expr = sx.make_union(sx.make_input(a), sx.make_input(b))
+    r1 = sx.evaluate(expr, ws)
+    p1 = r1.begin.data_ptr()
+    del r1
+    r2 = sx.evaluate(expr, ws)
+    assert r2.begin.data_ptr() == p1
+    # nested expressions with a workspace stay correct (no shared offsets)
+    nested = sx.make_union(sx.make_difference(sx.make_input(a), sx.make_input(b)),
+                           sx.make_intersection(sx.make_input(a), sx.make_input(b)))
+    got = to_host(sx.evaluate(nested, ws))
+    ha, hb = to_host(a), to_host(b)
+    assert_set_equal(got, ox.union(ox.difference(ha, hb), ox.intersection(ha, hb)))
+
+
+def test_invalid_inputs_raise(sx):
+    import torch
+
+    with pytest.raises(TypeError):
+        sx.IntervalSet(begin=torch.zeros(1, dtype=torch.int64), end=torch.zeros(1, dtype=torch.int32),
+                       row_offsets=torch.tensor([0, 1], dtype=torch.int32), rows=torch.tensor([0], dtype=torch.int32))
+    with pytest.raises(ValueError):
+        sx.build_interval_set([0, 1, 2], [0, 1], [1, 2], rows=[3, 3])
+    with pytest.raises(ValueError):
+        sx.build_interval_set([0, 2], [0], [1])
