@@ -1,0 +1,193 @@
+/*
+ * subsetix_hip.h -- C-ABI of libsubsetix_hip.so (gfx950 / MI355X).
+ *
+ * Drop-in boundary for the subsetix interval-set AMR hot path.  The reference
+ * (sbstndb/subsetix) drives CUDA-C kernels embedded in Python strings through
+ * cupy.RawKernel(src, name)(grid, block, args) with raw device pointers and
+ * numpy scalars; each entry point below replaces one of those kernels (or the
+ * CuPy glue around it) and cites the reference code it stands for.
+ *
+ * Conventions
+ *   - Interval sets are CSR: row_offsets[R+1] (int32, starts at 0), begin/end
+ *     [N] int32 half-open, rows[R] strictly increasing int32 row ids.
+ *   - All pointers are device pointers (HIP); sizes are int64_t.
+ *   - Every call is asynchronous on `stream` and returns 0 or a negative
+ *     status (STX_E*); stx_last_error() describes the last failure.
+ *   - Device-detected conditions (capacity overflow, misalignment, rows out of
+ *     range, look-back wait bound) are OR-ed into the caller's `status` word.
+ *   - The library never allocates device memory: outputs, temporaries and the
+ *     look-back `scratch` (zeroed once by the caller, left zeroed by every
+ *     launch) are caller buffers.  Calls sharing a scratch buffer must be
+ *     ordered (same stream).
+ */
+#ifndef SUBSETIX_HIP_H
+#define SUBSETIX_HIP_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    STX_OK = 0,
+    STX_EINVAL = -1,
+    STX_ECAPACITY = -2,
+    STX_EALIGN = -3,
+    STX_EHIP = -4,
+    STX_ESPIN = -5
+};
+
+/* status word bits */
+enum { STX_FLAG_CAPACITY = 1, STX_FLAG_SPIN = 2, STX_FLAG_ALIGN = 4, STX_FLAG_ROWS = 8 };
+
+/* operand transforms applied on load (set algebra / merges) */
+enum { STX_XF_NONE = 0, STX_XF_WIDEN = 1, STX_XF_COARSEN = 2, STX_XF_CLIP = 3, STX_XF_SCALE = 4 };
+
+/* ---- housekeeping ------------------------------------------------------- */
+int32_t stx_abi_version(void);
+const char* stx_last_error(void);
+void stx_set_spin_limit(int64_t polls);
+
+/* ---- scans ---------------------------------------------------------------
+ * Replaces cp.cumsum over counts (expressions.py:502, morphology.py:60,84,
+ * multilevel.py:261,316,340, interval_field.py:73, regrid.py:314,
+ * fields.py:94).  out[k] = sum(in[0..k)), out[n] = total (n = *d_n when
+ * d_n != NULL, else n_bound). */
+int32_t stx_scan_i32(const int32_t* in, const int32_t* d_n, int64_t n_bound, int32_t* out, int32_t* d_total,
+                     uint32_t* status, void* scratch, int64_t scratch_bytes, hipStream_t stream);
+int64_t stx_scan_scratch_bytes(int64_t n_bound);
+
+/* ---- set algebra ---------------------------------------------------------
+ * Replaces count_segments / write_segments (kernels.py:11-236) driven by
+ * _apply_binary (expressions.py:446-528), and the merge rebuild
+ * count/write_merged_segments (kernels.py:292-381) when b_off == NULL
+ * (op must then be 0).  op: 0 union, 1 intersection, 2 difference
+ * (expressions.py:529).  One launch: count, scan and write (single pass,
+ * decoupled look-back).  Operand X is read through an optional row map
+ * x_map[output row] -> source row (-1 = empty; replaces the re-indexing of
+ * _align_interval_sets, expressions.py:58-141) and an optional transform
+ * (STX_XF_*).  Output: out_off[U+1], out_beg/out_end (capacity entries),
+ * *d_total.  U = *d_nrows when non-NULL, else nrows_bound.
+ * group <= 0: automatic schedule (LDS tile path when no maps are given);
+ * 4 / 16 / 64: lane-group size of the per-row path. */
+int32_t stx_setop(int32_t op, const int32_t* a_beg, const int32_t* a_end, const int32_t* a_off, const int32_t* a_map,
+                  int32_t a_xf_mode, int32_t a_xf_param, int64_t a_n, const int32_t* b_beg, const int32_t* b_end,
+                  const int32_t* b_off, const int32_t* b_map, int32_t b_xf_mode, int32_t b_xf_param, int64_t b_n,
+                  const int32_t* d_nrows, int64_t nrows_bound, int32_t group, int32_t* out_off, int32_t* out_beg,
+                  int32_t* out_end, int64_t capacity, int32_t* d_total, uint32_t* status, void* scratch,
+                  int64_t scratch_bytes, hipStream_t stream);
+int64_t stx_setop_scratch_bytes(int64_t nrows_bound, int32_t group);
+
+/* Sorted union of two row-id lists (shifted) with source maps: replaces the
+ * cp.union1d + searchsorted of _align_interval_sets / _reindex_interval_set
+ * (expressions.py:58-141, :138).  tmp: 2*(nb+1) int32; outputs sized na+nb;
+ * *d_nu = union size. */
+int32_t stx_rows_union(const int32_t* rows_a, int64_t na, int32_t shift_a, const int32_t* rows_b, int64_t nb,
+                       int32_t shift_b, int32_t* rows_u, int32_t* map_a, int32_t* map_b, int32_t* d_nu, int32_t* tmp,
+                       uint32_t* status, void* scratch, int64_t scratch_bytes, hipStream_t stream);
+
+/* ---- row structure -------------------------------------------------------
+ * CSR checks of IntervalSet.__post_init__ (expressions.py:166-227): flag
+ * bits 1 off[0]!=0, 2 off[R]!=N, 4 rows not increasing, 8 offsets
+ * decreasing, 16 rows != arange(R). */
+int32_t stx_validate_csr(const int32_t* off, int64_t R, const int32_t* rows, int64_t N, uint32_t* flags,
+                         hipStream_t stream);
+/* IntervalSet.interval_rows (expressions.py:236-249): row id and/or row
+ * ordinal of every interval. */
+int32_t stx_interval_rows(const int32_t* off, int64_t R, const int32_t* rows, int64_t N, int32_t* out_row,
+                          int32_t* out_ord, hipStream_t stream);
+/* Dense row map for clip_interval_set (morphology.py:268-319): map[y] =
+ * index of row y0+y in rows, or -1. */
+int32_t stx_rows_lookup(const int32_t* rows, int64_t R, int32_t y0, int64_t H, int32_t* map, hipStream_t stream);
+/* Drop rows without intervals (the unique(rows) of the rebuild,
+ * morphology.py:43-48).  tmp: 2*(R+1) int32. */
+int32_t stx_compact_rows(const int32_t* rows, const int32_t* off, int64_t R, int32_t shift, int32_t* rows_out,
+                         int32_t* off_out, int32_t* d_count, int32_t* tmp, uint32_t* status, void* scratch,
+                         int64_t scratch_bytes, hipStream_t stream);
+/* Coarse rows floor(row/ratio) of restrict_set (multilevel.py:284-316) and
+ * per-phase fine source rows maps[ph*cap + c]. */
+int32_t stx_coarse_rows(const int32_t* rows, const int32_t* off, int64_t R, int32_t ratio, int32_t all_rows,
+                        int32_t* crow, int32_t* maps, int64_t cap, int32_t* d_count, int32_t* tmp, uint32_t* status,
+                        void* scratch, int64_t scratch_bytes, hipStream_t stream);
+/* _prolong_set_impl (multilevel.py:213-264) of a set whose rows all hold
+ * intervals: K*ratio rows, N*ratio intervals; fbase (optional) maps each fine
+ * interval to its coarse interval. */
+int32_t stx_prolong_set(const int32_t* rows, const int32_t* off, int64_t K, const int32_t* beg, const int32_t* end,
+                        int64_t N, int32_t ratio, int32_t* frows, int32_t* foff, int32_t* fbeg, int32_t* fend,
+                        int32_t* fbase, hipStream_t stream);
+/* Rows of the prolongation of an empty set (multilevel.py:225-232). */
+int32_t stx_prolong_rows(const int32_t* rows, int64_t K, int32_t ratio, int32_t* frows, hipStream_t stream);
+int32_t stx_fill_i32(int32_t* p, int64_t n, int32_t value, int32_t step, hipStream_t stream);
+/* translate_interval_set (morphology.py:115-140): dst = src + v. */
+int32_t stx_add_i32(const int32_t* src, int32_t* dst, int64_t n, int32_t v, hipStream_t stream);
+
+/* ---- fields -------------------------------------------------------------- */
+/* Interval lengths for create_interval_field's cell offsets
+ * (interval_field.py:64-75; scan them with stx_scan_i32). */
+int32_t stx_interval_lengths(const int32_t* beg, const int32_t* end, int64_t n, int32_t* len, hipStream_t stream);
+/* esize 1/4/8 fill of n elements with the bit pattern `bits`. */
+int32_t stx_fill_value(int32_t esize, void* p, int64_t n, int64_t bits, hipStream_t stream);
+/* prolong_field_f32 / _f64 (kernels.py:384-455; multilevel.py:365-407) for
+ * a coarse field without empty rows: K rows (c_off), cell offsets c_co.
+ * dtype 4 float32, 8 float64.  group: lane-group size (4/16/64). */
+int32_t stx_prolong_field(int32_t dtype, int32_t group, const int32_t* c_off, int64_t K, const int32_t* c_co,
+                          const void* cv, int32_t ratio, void* fv, hipStream_t stream);
+/* Alignment checks of restrict_field (multilevel.py:169-210, :433-451);
+ * sets STX_FLAG_ALIGN. */
+int32_t stx_restrict_align(const int32_t* c_rows, const int32_t* c_off, int64_t K, const int32_t* c_beg,
+                           const int32_t* c_end, const int32_t* f_rows, const int32_t* f_off, const int32_t* f_beg,
+                           const int32_t* f_end, int32_t ratio, uint32_t* status, hipStream_t stream);
+/* restrict_field_f32 / _f64 (kernels.py:458-598).  kind 0: float in, float
+ * accumulate (the reference kernel); 1: float in, double accumulate, float
+ * out; 2: double.  reducer 0 mean, 1 sum, 2 min, 3 max (multilevel.py:44). */
+int32_t stx_restrict_field(int32_t kind, int32_t group, const int32_t* c_off, int64_t K, const int32_t* c_co,
+                           const int32_t* f_off, const int32_t* f_co, const void* fv, int32_t ratio, int32_t reducer,
+                           void* cv, hipStream_t stream);
+/* Row ids within [0, height) (interval_stencil.py:183-191); STX_FLAG_ROWS. */
+int32_t stx_rows_in_range(const int32_t* rows, int64_t R, int32_t height, uint32_t* status, hipStream_t stream);
+/* interval_upwind + sample_interval_cell (interval_stencil.py:16-113):
+ * zero-exterior first-order upwind.  dtype 4: float arithmetic (scalars
+ * rounded to float), 8: double. */
+int32_t stx_upwind(int32_t dtype, int32_t group, const int32_t* rows, const int32_t* off, int64_t R, const int32_t* beg,
+                   const int32_t* end, const int32_t* co, const void* u, void* out, int32_t width, int32_t height,
+                   double a, double b, double dt, double dx, double dy, hipStream_t stream);
+/* FP64 mass sum(u)*scale with a fixed reduction order (no reference
+ * counterpart; conservation check of the AMR step).  part: stx_mass_parts(n)
+ * doubles. */
+int64_t stx_mass_parts(int64_t n);
+int32_t stx_mass(int32_t dtype, const void* u, int64_t n, double scale, double* part, double* out, hipStream_t stream);
+/* Row location for gather/scatter (fields.py:310-318): per subset interval,
+ * the index of its row in the super rows (-1 -> STX_FLAG_ROWS). */
+int32_t stx_locate_rows(const int32_t* sub_rows, const int32_t* sub_off, int64_t Rs, const int32_t* sup_rows, int64_t Rp,
+                        int32_t* srcrow_per_iv, uint32_t* status, hipStream_t stream);
+/* subsetix_interval_subset_gather / _scatter (fields.py:127-294).  mode 0
+ * gather (super -> subset), 1 scatter; esize 1, 4 or 8. */
+int32_t stx_subset_copy(int32_t mode, int32_t esize, int32_t group, const int32_t* sub_beg, const int32_t* sub_end,
+                        const int32_t* sub_co, const int32_t* sub_srcrow, int64_t nsub, const int32_t* sup_off,
+                        const int32_t* sup_beg, const int32_t* sup_end, const int32_t* sup_co, void* sup_v, void* sub_v,
+                        hipStream_t stream);
+/* _interval_field_value_intervals (fields.py:41-100): runs of cells equal to
+ * `target_bits` (esize 1 or 4), never across an interval. */
+int32_t stx_value_runs(int32_t esize, int32_t group, const int32_t* off, int64_t R, const int32_t* beg, const int32_t* co,
+                       const void* values, int64_t target_bits, int32_t* out_off, int32_t* out_beg, int32_t* out_end,
+                       int64_t capacity, int32_t* d_total, uint32_t* status, void* scratch, int64_t scratch_bytes,
+                       hipStream_t stream);
+
+/* ---- regrid -------------------------------------------------------------- */
+/* subsetix_count/write_threshold_intervals (regrid.py:20-124) and, with
+ * gradient != 0, subsetix_gradient_magnitude_flat (regrid.py:175-223) fused:
+ * maximal runs of v >= t per row of a dense width x height float field. */
+int32_t stx_threshold(int32_t gradient, const float* data, int32_t width, int32_t height, float t, int32_t* out_off,
+                      int32_t* out_beg, int32_t* out_end, int64_t capacity, int32_t* d_total, uint32_t* status,
+                      void* scratch, int64_t scratch_bytes, hipStream_t stream);
+int64_t stx_threshold_scratch_bytes(int32_t height);
+/* subsetix_gradient_magnitude_flat (regrid.py:175-223) as a separate pass. */
+int32_t stx_gradient(const float* data, int32_t width, int32_t height, float* out, hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SUBSETIX_HIP_H */

@@ -19,6 +19,33 @@ from .expressions import (
     make_symmetric_difference,
     make_union,
 )
+from .interval_field import IntervalField, create_interval_field, get_cell, set_cell
+from .interval_stencil import step_upwind_interval, step_upwind_interval_field
+from .morphology import (
+    boundary_for_direction,
+    boundary_layer,
+    clip_interval_set,
+    dilate_interval_set,
+    erode_interval_set,
+    full_interval_set,
+    ghost_zones,
+    interior_for_direction,
+    translate_interval_set,
+)
+from .multilevel import (
+    MultiLevel2D,
+    MultiLevelField2D,
+    coarse_only,
+    covered_by_fine,
+    prolong_field,
+    prolong_level_field,
+    prolong_level_sets,
+    prolong_set,
+    restrict_field,
+    restrict_level_field,
+    restrict_level_sets,
+    restrict_set,
+)
 
 __all__ = [
     "Expr",
@@ -32,4 +59,31 @@ __all__ = [
     "make_intersection",
     "make_symmetric_difference",
     "make_union",
+    "IntervalField",
+    "create_interval_field",
+    "get_cell",
+    "set_cell",
+    "MultiLevel2D",
+    "MultiLevelField2D",
+    "prolong_set",
+    "restrict_set",
+    "prolong_field",
+    "restrict_field",
+    "prolong_level_sets",
+    "restrict_level_sets",
+    "prolong_level_field",
+    "restrict_level_field",
+    "covered_by_fine",
+    "coarse_only",
+    "translate_interval_set",
+    "interior_for_direction",
+    "boundary_for_direction",
+    "boundary_layer",
+    "dilate_interval_set",
+    "erode_interval_set",
+    "ghost_zones",
+    "full_interval_set",
+    "clip_interval_set",
+    "step_upwind_interval",
+    "step_upwind_interval_field",
 ]

@@ -125,13 +125,15 @@ def setop_raw(op: int, a, b, *, nrows: int, d_nrows: Optional[torch.Tensor] = No
         ob, oe = buffers
     else:
         ob, oe = empty_i32(cap, dev), empty_i32(cap, dev)
-    g = group or _GROUP_OVERRIDE or choose_group(na + nb, nrows)
+    g = group or _GROUP_OVERRIDE or 0
+    if g == 0 and (map_a is not None or map_b is not None or d_nrows is not None):
+        g = choose_group(na + nb, nrows)
     nbytes = _lib.lib().stx_setop_scratch_bytes(nrows, g)
     sp, sbytes = ctx.scratch(nbytes)
     b_beg, b_end, b_off = b if b is not None else (None, None, None)
     check(_lib.lib().stx_setop(int(op), ptr(a_beg), ptr(a_end), ptr(a_off), ptr(map_a), int(xf_a[0]), int(xf_a[1]),
-                               ptr(b_beg), ptr(b_end), ptr(b_off), ptr(map_b), int(xf_b[0]), int(xf_b[1]),
-                               ptr(d_nrows), int(nrows), int(g), ptr(out_off), ptr(ob), ptr(oe), cap,
+                               int(na), ptr(b_beg), ptr(b_end), ptr(b_off), ptr(map_b), int(xf_b[0]), int(xf_b[1]),
+                               int(nb), ptr(d_nrows), int(nrows), int(g), ptr(out_off), ptr(ob), ptr(oe), cap,
                                ctx.slot(0), ctx.status_ptr, sp, sbytes, ctx.stream), "setop")
     return out_off, ob, oe, ctx
 

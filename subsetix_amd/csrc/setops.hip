@@ -189,6 +189,325 @@ struct SetopRow {
     }
 };
 
+
+// ---------------------------------------------------------------------------
+// Tile kernel (identity row maps): LDS-staged merge path.
+//
+// A block owns RPT consecutive rows.  Their A and B intervals are contiguous
+// in memory; the block copies them (transformed keys) into LDS with
+// coalesced loads.  The tile's merged positions (row by row, A and B merged
+// by key) are split evenly: thread t owns positions [t*VT, t*VT + VT).  A
+// thread finds its first element with one binary search (row) and one
+// co-rank search (inside the row), then walks its positions with a
+// two-pointer merge, keeping each position's result in registers.  Union
+// needs the largest end before each position (within the row): a segmented
+// max-scan over the threads' aggregates.  Emission counts are scanned over
+// the block, the tile offset comes from the decoupled look-back, and the
+// intervals are written in tile order (contiguous).  A tile whose rows hold
+// more than PCAP intervals falls back to one wavefront per row.
+constexpr int kVT = 8;
+constexpr int kPCAP = kBlock * kVT;
+constexpr int kMaxRPT = 256;
+
+struct SegMax {
+    int32_t v;
+    int32_t f;
+};
+
+__device__ __forceinline__ SegMax seg_combine(SegMax a, SegMax b) {  // a before b
+    SegMax r;
+    r.f = a.f | b.f;
+    r.v = b.f ? b.v : (a.v > b.v ? a.v : b.v);
+    return r;
+}
+
+// Block-wide exclusive segmented max-scan (one element per thread).
+__device__ __forceinline__ SegMax block_seg_max_exclusive(SegMax x, SegMax* lds /* >= kWaves */) {
+    const int lane = lane_id(), wave = wave_id();
+    SegMax inc = x;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        SegMax o;
+        o.v = __shfl_up(inc.v, d, 64);
+        o.f = __shfl_up(inc.f, d, 64);
+        if (lane >= d) inc = seg_combine(o, inc);
+    }
+    __syncthreads();
+    if (lane == 63) lds[wave] = inc;
+    __syncthreads();
+    SegMax carry{INT_MIN, 0};
+    for (int w = 0; w < wave; ++w) carry = seg_combine(carry, lds[w]);
+    SegMax ex;
+    ex.v = __shfl_up(inc.v, 1, 64);
+    ex.f = __shfl_up(inc.f, 1, 64);
+    if (lane == 0) ex = SegMax{INT_MIN, 0};
+    return seg_combine(carry, ex);
+}
+
+template <int OP>
+__device__ __forceinline__ bool key_a_first(int32_t akb, int32_t bkb, int32_t bke) {
+    if (OP == 2) return akb < bke;
+    return akb <= bkb;
+}
+
+template <int OP>
+__global__ __launch_bounds__(kBlock) void setop_tile_kernel(Operand A, Operand B, int64_t U, int32_t RPT, uint32_t ntiles,
+                                                            int32_t* __restrict__ out_off, int32_t* __restrict__ ob,
+                                                            int32_t* __restrict__ oe, int64_t capacity,
+                                                            int32_t* __restrict__ d_total, uint32_t* status,
+                                                            void* scratch, int64_t spin_limit) {
+    __shared__ int32_t s_kb[kPCAP];
+    __shared__ int32_t s_ke[kPCAP];
+    __shared__ int32_t s_ex[kPCAP + 1];
+    __shared__ int32_t s_aS[kMaxRPT + 1];
+    __shared__ int32_t s_bS[kMaxRPT + 1];
+    __shared__ int32_t s_M[kMaxRPT + 1];
+    __shared__ int32_t s_scan[17];
+    __shared__ SegMax s_seg[kWaves];
+    __shared__ uint32_t s_slot;
+    __shared__ uint32_t s_excl;
+
+    LookbackScratch lb = lookback_scratch(scratch, ntiles);
+    const uint32_t tile = lookback_ticket(lb, &s_slot);
+    const int tid = threadIdx.x;
+    const int64_t row0 = (int64_t)tile * RPT;
+    const int nr = (int)(U - row0 < RPT ? (U - row0 > 0 ? U - row0 : 0) : RPT);
+
+    const int32_t a_base = nr > 0 ? A.off[row0] : 0;
+    const int32_t b_base = (nr > 0 && B.off) ? B.off[row0] : 0;
+    for (int r = tid; r <= nr; r += kBlock) {
+        int32_t as = A.off[row0 + r] - a_base;
+        int32_t bs = B.off ? B.off[row0 + r] - b_base : 0;
+        s_aS[r] = as;
+        s_bS[r] = bs;
+        s_M[r] = as + bs;
+    }
+    __syncthreads();
+    const int32_t nA = s_aS[nr], nB = s_bS[nr];
+    const int32_t P = nA + nB;
+    const bool fits = P <= kPCAP;
+
+    int32_t tile_total = 0;
+    // per-position results (registers)
+    int32_t rb[kVT], re[kVT];
+    uint32_t emit_mask = 0, start_mask = 0, alive_mask = 0, rowstart_mask = 0, rowend_mask = 0;
+    SegMax carry{INT_MIN, 0};
+    int32_t my_base = 0;
+
+    if (fits) {
+        for (int k = tid; k < nA; k += kBlock) {
+            s_kb[k] = keyb(A, a_base + k);
+            s_ke[k] = keye(A, a_base + k);
+        }
+        for (int k = tid; k < nB; k += kBlock) {
+            s_kb[nA + k] = keyb(B, b_base + k);
+            s_ke[nA + k] = keye(B, b_base + k);
+        }
+        __syncthreads();
+        const int32_t p0 = tid * kVT;
+        if (p0 < P) {
+            // row of p0
+            int lo = 0, hi = nr;
+            while (hi - lo > 1) {
+                int mid = (lo + hi) >> 1;
+                if (s_M[mid] <= p0) lo = mid; else hi = mid;
+            }
+            int r = lo;
+            int32_t aS = s_aS[r], na = s_aS[r + 1] - aS;
+            int32_t bS = nA + s_bS[r], nb = s_bS[r + 1] - s_bS[r];
+            int32_t rowend = s_M[r + 1];
+            // co-rank inside the row
+            int32_t l = p0 - s_M[r];
+            int32_t clo = l - nb > 0 ? l - nb : 0, chi = l < na ? l : na;
+            while (clo < chi) {
+                int32_t mid = (clo + chi) >> 1;
+                int32_t bj = bS + (l - 1 - mid);
+                if (key_a_first<OP>(s_kb[aS + mid], s_kb[bj], s_ke[bj])) clo = mid + 1;
+                else chi = mid;
+            }
+            int32_t i = clo, j = l - clo;
+#pragma unroll
+            for (int v = 0; v < kVT; ++v) {
+                const int32_t p = p0 + v;
+                rb[v] = 0;
+                re[v] = INT_MIN;
+                if (p < P) {
+                    while (p >= rowend) {  // next non-empty row
+                        ++r;
+                        aS = s_aS[r];
+                        na = s_aS[r + 1] - aS;
+                        bS = nA + s_bS[r];
+                        nb = s_bS[r + 1] - s_bS[r];
+                        rowend = s_M[r + 1];
+                        i = 0;
+                        j = 0;
+                    }
+                    if (p == s_M[r]) rowstart_mask |= 1u << v;
+                    if (p == rowend - 1) rowend_mask |= 1u << v;
+                    const bool isA = (j >= nb) || (i < na && key_a_first<OP>(s_kb[aS + i], s_kb[bS + j], s_ke[bS + j]));
+                    if (OP == 0) {
+                        int32_t eb, ee;
+                        bool clip;
+                        if (isA) { eb = s_kb[aS + i]; ee = s_ke[aS + i]; clip = A.xf.mode == XF_CLIP; }
+                        else { eb = s_kb[bS + j]; ee = s_ke[bS + j]; clip = B.xf.mode == XF_CLIP; }
+                        if (!(clip && eb >= ee)) alive_mask |= 1u << v;
+                        rb[v] = eb;
+                        re[v] = ee;
+                    } else if (OP == 1) {
+                        if (isA) {
+                            const int32_t ab = s_kb[aS + i], ae = s_ke[aS + i];
+                            int32_t ce = INT_MIN;
+                            if (j >= 1 && s_ke[bS + j - 1] > ab) ce = s_ke[bS + j - 1];
+                            else if (j < nb && s_kb[bS + j] == ab) ce = s_ke[bS + j];
+                            if (ce != INT_MIN) { emit_mask |= 1u << v; rb[v] = ab; re[v] = ae < ce ? ae : ce; }
+                        } else {
+                            const int32_t bb = s_kb[bS + j], be = s_ke[bS + j];
+                            if (i >= 1) {
+                                const int32_t pab = s_kb[aS + i - 1], pae = s_ke[aS + i - 1];
+                                if (pab < bb && bb < pae) { emit_mask |= 1u << v; rb[v] = bb; re[v] = be < pae ? be : pae; }
+                            }
+                        }
+                    } else {
+                        if (isA) {
+                            const int32_t ab = s_kb[aS + i], ae = s_ke[aS + i];
+                            const bool covered = j < nb && s_kb[bS + j] <= ab;
+                            if (!covered) {
+                                int32_t pe = ae;
+                                if (j < nb) { const int32_t nbg = s_kb[bS + j]; pe = pe < nbg ? pe : nbg; }
+                                emit_mask |= 1u << v;
+                                rb[v] = ab;
+                                re[v] = pe;
+                            }
+                        } else {
+                            const int32_t be = s_ke[bS + j];
+                            if (i >= 1) {
+                                const int32_t pae = s_ke[aS + i - 1];
+                                const bool open = (j + 1 >= nb) || s_kb[bS + j + 1] > be;
+                                if (be < pae && open) {
+                                    int32_t pe = pae;
+                                    if (j + 1 < nb) { const int32_t nbg = s_kb[bS + j + 1]; pe = pe < nbg ? pe : nbg; }
+                                    emit_mask |= 1u << v;
+                                    rb[v] = be;
+                                    re[v] = pe;
+                                }
+                            }
+                        }
+                    }
+                    if (isA) ++i; else ++j;
+                }
+            }
+        }
+        if (OP == 0) {
+            // thread aggregate: max end since the last row start in the thread
+            SegMax agg{INT_MIN, 0};
+#pragma unroll
+            for (int v = 0; v < kVT; ++v) {
+                if (rowstart_mask & (1u << v)) agg = SegMax{INT_MIN, 1};
+                if (alive_mask & (1u << v)) agg.v = agg.v > re[v] ? agg.v : re[v];
+            }
+            carry = block_seg_max_exclusive(agg, s_seg);
+            int32_t pm = carry.v;
+#pragma unroll
+            for (int v = 0; v < kVT; ++v) {
+                if (rowstart_mask & (1u << v)) pm = INT_MIN;
+                if ((alive_mask & (1u << v)) && rb[v] > pm) start_mask |= 1u << v;
+                if (alive_mask & (1u << v)) pm = pm > re[v] ? pm : re[v];
+            }
+            emit_mask = start_mask;
+        }
+        const int32_t my_count = __popc(emit_mask);
+        my_base = block_exclusive_scan<int32_t>(my_count, s_scan, &tile_total);
+        {
+            int32_t run = my_base;
+            const int32_t p0 = tid * kVT;
+#pragma unroll
+            for (int v = 0; v < kVT; ++v) {
+                if (p0 + v < P) s_ex[p0 + v] = run;
+                if (emit_mask & (1u << v)) ++run;
+            }
+            if (tid == 0) s_ex[P] = tile_total;
+        }
+    } else {
+        // fallback: one wavefront per row, counts into s_ex[r]
+        const int wave = wave_id();
+        for (int r = wave; r < nr; r += kWaves) {
+            int32_t a0, na, b0, nb;
+            row_extent(A, row0 + r, a0, na);
+            row_extent(B, row0 + r, b0, nb);
+            int32_t c = setop_row<64, OP, false>(A, a0, na, B, b0, nb, nullptr, nullptr, 0);
+            if (lane_id() == 0) s_ex[r] = c;
+        }
+        __syncthreads();
+        int32_t v = tid < nr ? s_ex[tid] : 0;
+        int32_t ex = 0;
+        // nr <= kMaxRPT == kBlock
+        ex = block_exclusive_scan<int32_t>(v, s_scan, &tile_total);
+        __syncthreads();
+        if (tid < nr) s_ex[tid] = ex;
+        if (tid == 0) s_ex[nr] = tile_total;
+    }
+    if (wave_id() == 0) {
+        uint32_t e = lookback_exclusive(lb, tile, (uint32_t)tile_total, status, spin_limit);
+        if (lane_id() == 0) s_excl = e;
+    }
+    __syncthreads();
+    const int64_t excl = s_excl;
+    // row offsets
+    for (int r = tid; r <= nr; r += kBlock) {
+        if (r == nr && row0 + nr != U) break;
+        int32_t local = fits ? s_ex[s_M[r]] : s_ex[r];
+        out_off[row0 + r] = (int32_t)(excl + local);
+        if (r == nr && d_total) *d_total = (int32_t)(excl + local);
+    }
+    if (nr == 0 && row0 == U && tid == 0) {
+        out_off[U] = (int32_t)excl;
+        if (d_total) *d_total = (int32_t)excl;
+    }
+    if (excl + tile_total > capacity) {
+        if (tid == 0 && tile_total > 0) atomicOr(status, STX_FLAG_CAPACITY);
+    } else if (fits) {
+        int64_t w = excl + my_base;
+        if (OP == 0) {
+            // starts write begins; a start that is not the first of its row
+            // closes the previous run; the last position of a row closes the
+            // row's last run.
+            int32_t pm = carry.v;
+            const int32_t p0 = tid * kVT;
+#pragma unroll
+            for (int v = 0; v < kVT; ++v) {
+                if (p0 + v >= P) break;
+                if (rowstart_mask & (1u << v)) pm = INT_MIN;
+                const bool st = start_mask & (1u << v);
+                if (st) {
+                    ob[w] = rb[v];
+                    if (pm != INT_MIN) oe[w - 1] = pm;
+                    ++w;
+                }
+                if (alive_mask & (1u << v)) pm = pm > re[v] ? pm : re[v];
+                if ((rowend_mask & (1u << v)) && pm != INT_MIN) oe[w - 1] = pm;
+            }
+        } else {
+#pragma unroll
+            for (int v = 0; v < kVT; ++v) {
+                if (emit_mask & (1u << v)) {
+                    ob[w] = rb[v];
+                    oe[w] = re[v];
+                    ++w;
+                }
+            }
+        }
+    } else {
+        const int wave = wave_id();
+        for (int r = wave; r < nr; r += kWaves) {
+            int32_t a0, na, b0, nb;
+            row_extent(A, row0 + r, a0, na);
+            row_extent(B, row0 + r, b0, nb);
+            setop_row<64, OP, true>(A, a0, na, B, b0, nb, ob, oe, excl + s_ex[r]);
+        }
+    }
+    lookback_finish(lb, ntiles, &s_slot);
+}
+
 // ---------------------------------------------------------------------------
 // Generic exclusive scan of int32 counts (single pass, decoupled look-back).
 // out[k] = sum(in[0..k)), out[n] = total; *d_total = total.
@@ -301,6 +620,17 @@ __global__ void add_i32_kernel(int32_t* p, int64_t n, int32_t v) {
 // Host side (C-ABI).
 #include "stx_host.h"
 
+// Rows per tile for the LDS tile kernel: about 3/4 of the tile capacity at
+// the mean row length (the row-bucket schedule of the tile path).
+static int32_t setop_tile_rows(int64_t elems, int64_t rows) {
+    double avg = rows > 0 ? (double)elems / (double)rows : 1.0;
+    if (avg < 1.0) avg = 1.0;
+    int64_t r = (int64_t)(0.75 * kPCAP / avg);
+    if (r < 1) r = 1;
+    if (r > kMaxRPT) r = kMaxRPT;
+    return (int32_t)r;
+}
+
 template <int G>
 static int32_t launch_setop_g(int32_t op, const Operand& A, const Operand& B, const int32_t* d_nrows, int64_t nrows_bound,
                               int32_t* out_off, int32_t* ob, int32_t* oe, int64_t capacity, int32_t* d_total,
@@ -330,24 +660,47 @@ int64_t stx_scan_scratch_bytes(int64_t n_bound) {
 }
 
 int64_t stx_setop_scratch_bytes(int64_t nrows_bound, int32_t group) {
-    return fused_rows_scratch_bytes(nrows_bound, group);
+    // the tile path never uses more tiles than the 4-lane row-group path
+    (void)group;
+    return fused_rows_scratch_bytes(nrows_bound, 4) > lookback_bytes(nrows_bound + 1)
+               ? fused_rows_scratch_bytes(nrows_bound, 4)
+               : lookback_bytes(nrows_bound + 1);
 }
 
 int32_t stx_setop(int32_t op, const int32_t* a_beg, const int32_t* a_end, const int32_t* a_off, const int32_t* a_map,
-                  int32_t a_xf_mode, int32_t a_xf_param, const int32_t* b_beg, const int32_t* b_end, const int32_t* b_off,
-                  const int32_t* b_map, int32_t b_xf_mode, int32_t b_xf_param, const int32_t* d_nrows, int64_t nrows_bound,
-                  int32_t group, int32_t* out_off, int32_t* out_beg, int32_t* out_end, int64_t capacity,
-                  int32_t* d_total, uint32_t* status, void* scratch, int64_t scratch_bytes, hipStream_t stream) {
+                  int32_t a_xf_mode, int32_t a_xf_param, int64_t a_n, const int32_t* b_beg, const int32_t* b_end,
+                  const int32_t* b_off, const int32_t* b_map, int32_t b_xf_mode, int32_t b_xf_param, int64_t b_n,
+                  const int32_t* d_nrows, int64_t nrows_bound, int32_t group, int32_t* out_off, int32_t* out_beg,
+                  int32_t* out_end, int64_t capacity, int32_t* d_total, uint32_t* status, void* scratch,
+                  int64_t scratch_bytes, hipStream_t stream) {
     if (op < 0 || op > 2 || nrows_bound < 0 || a_off == nullptr) return stx_fail(STX_EINVAL, "setop: bad arguments");
     if (op != 0 && b_off == nullptr) return stx_fail(STX_EINVAL, "setop: intersection/difference need two operands");
     Operand A{a_beg, a_end, a_off, a_map, XForm{a_xf_mode, a_xf_param}};
     Operand B{b_beg, b_end, b_off, b_map, XForm{b_xf_mode, b_xf_param}};
     if ((a_xf_mode == XF_COARSEN && a_xf_param < 1) || (b_xf_mode == XF_COARSEN && b_xf_param < 1))
         return stx_fail(STX_EINVAL, "setop: coarsening ratio must be >= 1");
-    if (group >= 64)
+    const bool tile_path = a_map == nullptr && b_map == nullptr && d_nrows == nullptr && group <= 0;
+    if (tile_path) {
+        const int32_t rpt = setop_tile_rows(a_n + (b_off ? b_n : 0), nrows_bound);
+        const uint32_t ntiles = (uint32_t)((nrows_bound + rpt) / rpt);
+        if (lookback_bytes(ntiles) > scratch_bytes) return stx_fail(STX_EINVAL, "setop: scratch too small");
+        const int64_t spin = stx_spin_limit();
+        if (op == 0)
+            hipLaunchKernelGGL(setop_tile_kernel<0>, dim3(ntiles), dim3(kBlock), 0, stream, A, B, nrows_bound, rpt, ntiles,
+                               out_off, out_beg, out_end, capacity, d_total, status, scratch, spin);
+        else if (op == 1)
+            hipLaunchKernelGGL(setop_tile_kernel<1>, dim3(ntiles), dim3(kBlock), 0, stream, A, B, nrows_bound, rpt, ntiles,
+                               out_off, out_beg, out_end, capacity, d_total, status, scratch, spin);
+        else
+            hipLaunchKernelGGL(setop_tile_kernel<2>, dim3(ntiles), dim3(kBlock), 0, stream, A, B, nrows_bound, rpt, ntiles,
+                               out_off, out_beg, out_end, capacity, d_total, status, scratch, spin);
+        return stx_launch_status("setop_tile_kernel");
+    }
+    const int32_t g = group > 0 ? group : 64;
+    if (g >= 64)
         return launch_setop_g<64>(op, A, B, d_nrows, nrows_bound, out_off, out_beg, out_end, capacity, d_total, status,
                                   scratch, scratch_bytes, stream);
-    if (group >= 16)
+    if (g >= 16)
         return launch_setop_g<16>(op, A, B, d_nrows, nrows_bound, out_off, out_beg, out_end, capacity, d_total, status,
                                   scratch, scratch_bytes, stream);
     return launch_setop_g<4>(op, A, B, d_nrows, nrows_bound, out_off, out_beg, out_end, capacity, d_total, status,

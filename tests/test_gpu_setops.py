@@ -55,7 +55,7 @@ def test_golden_cell_model(sx, golden):
             assert ox.cells(to_host(got)) == cell_model(spec, cells), spec
 
 
-@pytest.mark.parametrize("group", [None, 4, 16, 64])
+@pytest.mark.parametrize("group", [None, 4, 16, 64])  # None: LDS tile path
 @pytest.mark.parametrize("seed", [0, 1, 2])
 def test_random_vs_oracle(sx, seed, group):
     from subsetix_amd import _core
@@ -138,3 +138,35 @@ def test_invalid_inputs_raise(sx):
         sx.build_interval_set([0, 1, 2], [0, 1], [1, 2], rows=[3, 3])
     with pytest.raises(ValueError):
         sx.build_interval_set([0, 2], [0], [1])
+
+
+def test_tile_fallback_long_rows(sx):
+    """Rows whose intervals overflow the LDS tile (more than 2048 merged
+    intervals) take the per-row wavefront path inside the same launch."""
+    rng = np.random.default_rng(42)
+    for counts in ([3000, 2, 0, 5], [1, 1500, 1500, 0, 7, 2600], [0] * 30 + [5000]):
+        off, beg, end = [0], [], []
+        offb, begb, endb = [0], [], []
+        for c in counts:
+            x = 0
+            for _ in range(c):
+                b = x + int(rng.integers(1, 4))
+                e = b + int(rng.integers(1, 4))
+                beg.append(b)
+                end.append(e)
+                x = e
+            off.append(len(beg))
+            x = 0
+            for _ in range(c // 2 + 1):
+                b = x + int(rng.integers(1, 6))
+                e = b + int(rng.integers(1, 6))
+                begb.append(b)
+                endb.append(e)
+                x = e
+            offb.append(len(begb))
+        a = ox.make(off, beg, end)
+        b = ox.make(offb, begb, endb)
+        da, db = to_device(a), to_device(b)
+        for op in ("union", "intersection", "difference"):
+            print("long rows", counts[:3], op, flush=True)
+            assert_set_equal(to_host(_ops(sx)[op](da, db)), ox.setop(op, a, b))
