@@ -16,9 +16,11 @@
  *   - Device-detected conditions (capacity overflow, misalignment, rows out of
  *     range, look-back wait bound) are OR-ed into the caller's `status` word.
  *   - The library never allocates device memory: outputs, temporaries and the
- *     look-back `scratch` (zeroed once by the caller, left zeroed by every
- *     launch) are caller buffers.  Calls sharing a scratch buffer must be
- *     ordered (same stream).
+ *     look-back `scratch` are caller buffers.  The scratch is zeroed once by
+ *     the caller; every launch that uses it passes a new `epoch` (1, 2, ...,
+ *     < 2^30; re-zero the scratch before wrapping), so nothing is cleared
+ *     between launches.  Calls sharing a scratch buffer must be ordered (same
+ *     stream).
  */
 #ifndef SUBSETIX_HIP_H
 #define SUBSETIX_HIP_H
@@ -56,7 +58,7 @@ void stx_set_spin_limit(int64_t polls);
  * fields.py:94).  out[k] = sum(in[0..k)), out[n] = total (n = *d_n when
  * d_n != NULL, else n_bound). */
 int32_t stx_scan_i32(const int32_t* in, const int32_t* d_n, int64_t n_bound, int32_t* out, int32_t* d_total,
-                     uint32_t* status, void* scratch, int64_t scratch_bytes, hipStream_t stream);
+                     uint32_t* status, void* scratch, int64_t scratch_bytes, uint32_t epoch, hipStream_t stream);
 int64_t stx_scan_scratch_bytes(int64_t n_bound);
 
 /* ---- set algebra ---------------------------------------------------------
@@ -77,7 +79,7 @@ int32_t stx_setop(int32_t op, const int32_t* a_beg, const int32_t* a_end, const 
                   const int32_t* b_off, const int32_t* b_map, int32_t b_xf_mode, int32_t b_xf_param, int64_t b_n,
                   const int32_t* d_nrows, int64_t nrows_bound, int32_t group, int32_t* out_off, int32_t* out_beg,
                   int32_t* out_end, int64_t capacity, int32_t* d_total, uint32_t* status, void* scratch,
-                  int64_t scratch_bytes, hipStream_t stream);
+                  int64_t scratch_bytes, uint32_t epoch, hipStream_t stream);
 int64_t stx_setop_scratch_bytes(int64_t nrows_bound, int32_t group);
 
 /* Sorted union of two row-id lists (shifted) with source maps: replaces the
@@ -86,7 +88,7 @@ int64_t stx_setop_scratch_bytes(int64_t nrows_bound, int32_t group);
  * *d_nu = union size. */
 int32_t stx_rows_union(const int32_t* rows_a, int64_t na, int32_t shift_a, const int32_t* rows_b, int64_t nb,
                        int32_t shift_b, int32_t* rows_u, int32_t* map_a, int32_t* map_b, int32_t* d_nu, int32_t* tmp,
-                       uint32_t* status, void* scratch, int64_t scratch_bytes, hipStream_t stream);
+                       uint32_t* status, void* scratch, int64_t scratch_bytes, uint32_t epoch, hipStream_t stream);
 
 /* ---- row structure -------------------------------------------------------
  * CSR checks of IntervalSet.__post_init__ (expressions.py:166-227): flag
@@ -105,12 +107,12 @@ int32_t stx_rows_lookup(const int32_t* rows, int64_t R, int32_t y0, int64_t H, i
  * morphology.py:43-48).  tmp: 2*(R+1) int32. */
 int32_t stx_compact_rows(const int32_t* rows, const int32_t* off, int64_t R, int32_t shift, int32_t* rows_out,
                          int32_t* off_out, int32_t* d_count, int32_t* tmp, uint32_t* status, void* scratch,
-                         int64_t scratch_bytes, hipStream_t stream);
+                         int64_t scratch_bytes, uint32_t epoch, hipStream_t stream);
 /* Coarse rows floor(row/ratio) of restrict_set (multilevel.py:284-316) and
  * per-phase fine source rows maps[ph*cap + c]. */
 int32_t stx_coarse_rows(const int32_t* rows, const int32_t* off, int64_t R, int32_t ratio, int32_t all_rows,
                         int32_t* crow, int32_t* maps, int64_t cap, int32_t* d_count, int32_t* tmp, uint32_t* status,
-                        void* scratch, int64_t scratch_bytes, hipStream_t stream);
+                        void* scratch, int64_t scratch_bytes, uint32_t epoch, hipStream_t stream);
 /* _prolong_set_impl (multilevel.py:213-264) of a set whose rows all hold
  * intervals: K*ratio rows, N*ratio intervals; fbase (optional) maps each fine
  * interval to its coarse interval. */
@@ -173,15 +175,50 @@ int32_t stx_subset_copy(int32_t mode, int32_t esize, int32_t group, const int32_
 int32_t stx_value_runs(int32_t esize, int32_t group, const int32_t* off, int64_t R, const int32_t* beg, const int32_t* co,
                        const void* values, int64_t target_bits, int32_t* out_off, int32_t* out_beg, int32_t* out_end,
                        int64_t capacity, int32_t* d_total, uint32_t* status, void* scratch, int64_t scratch_bytes,
-                       hipStream_t stream);
+                       uint32_t epoch, hipStream_t stream);
+
+/* Batched upwind over a stack of independent instances: row k belongs to
+ * instance row_inst[k] (width inst_w[i], coefficients inst_coef[5i..5i+4] =
+ * a, b, dt, dx, dy in the field type); instance row ids are separated by
+ * gaps.  No reference counterpart (the reference advances one instance per
+ * call, simulation.py:330-354). */
+int32_t stx_upwind_batched(int32_t dtype, int32_t group, const int32_t* rows, const int32_t* off, int64_t R,
+                           const int32_t* beg, const int32_t* end, const int32_t* co, const void* u, void* out,
+                           const int32_t* row_inst, const int32_t* inst_w, const void* inst_coef, hipStream_t stream);
+/* Prolong (mode 0) / restrict (mode 1) onto the cells of a target set T that
+ * lie in destination geometry D, reading source geometry S: the restrict /
+ * gather / scatter and the gather / prolong / scatter chains of
+ * synchronize_interval_fields (fields.py:570-612) in one pass each.
+ * Geometries: rows, off, beg, end, cell offsets, row count. */
+int32_t stx_transfer(int32_t mode, int32_t dtype, int32_t acc64, int32_t group, int32_t ratio, const int32_t* t_rows,
+                     const int32_t* t_off, const int32_t* t_beg, const int32_t* t_end, const int32_t* t_co, int64_t t_R,
+                     const int32_t* d_rows, const int32_t* d_off, const int32_t* d_beg, const int32_t* d_end,
+                     const int32_t* d_co, int64_t d_R, const int32_t* s_rows, const int32_t* s_off, const int32_t* s_beg,
+                     const int32_t* s_end, const int32_t* s_co, int64_t s_R, const void* src, void* dst,
+                     hipStream_t stream);
+
+/* Instance of every row of a stack (last i with bases[i] <= rows[k]). */
+int32_t stx_row_instance(const int32_t* rows, int64_t R, const int32_t* bases, int32_t ninst, int32_t* out,
+                         hipStream_t stream);
+/* FP64 mass per instance: out[i] = scale[i] * sum of u over cells
+ * [cstart[i], cstart[i+1]) with a fixed reduction order. */
+int32_t stx_instance_mass(int32_t dtype, const void* u, const int64_t* cstart, int32_t ninst, const double* scale,
+                          double* out, hipStream_t stream);
 
 /* ---- regrid -------------------------------------------------------------- */
+/* Gradient tagging on an interval field (generalises regrid.py:175-223 +
+ * 20-124 to sparse levels: one-sided differences where a neighbour is
+ * missing); runs of |grad u| >= t, per row of the field. */
+int32_t stx_grad_tag(int32_t dtype, int32_t group, const int32_t* rows, const int32_t* off, int64_t R, const int32_t* beg,
+                     const int32_t* end, const int32_t* co, const void* u, double t, int32_t* out_off, int32_t* out_beg,
+                     int32_t* out_end, int64_t capacity, int32_t* d_total, uint32_t* status, void* scratch,
+                     int64_t scratch_bytes, uint32_t epoch, hipStream_t stream);
 /* subsetix_count/write_threshold_intervals (regrid.py:20-124) and, with
  * gradient != 0, subsetix_gradient_magnitude_flat (regrid.py:175-223) fused:
  * maximal runs of v >= t per row of a dense width x height float field. */
 int32_t stx_threshold(int32_t gradient, const float* data, int32_t width, int32_t height, float t, int32_t* out_off,
                       int32_t* out_beg, int32_t* out_end, int64_t capacity, int32_t* d_total, uint32_t* status,
-                      void* scratch, int64_t scratch_bytes, hipStream_t stream);
+                      void* scratch, int64_t scratch_bytes, uint32_t epoch, hipStream_t stream);
 int64_t stx_threshold_scratch_bytes(int32_t height);
 /* subsetix_gradient_magnitude_flat (regrid.py:175-223) as a separate pass. */
 int32_t stx_gradient(const float* data, int32_t width, int32_t height, float* out, hipStream_t stream);

@@ -20,8 +20,8 @@ I32 = torch.int32
 _lib.register("stx_validate_csr", _I32, [_P, _I64, _P, _I64, _P, _P])
 _lib.register("stx_interval_rows", _I32, [_P, _I64, _P, _I64, _P, _P, _P])
 _lib.register("stx_rows_lookup", _I32, [_P, _I64, _I32, _I64, _P, _P])
-_lib.register("stx_compact_rows", _I32, [_P, _P, _I64, _I32, _P, _P, _P, _P, _P, _P, _I64, _P])
-_lib.register("stx_coarse_rows", _I32, [_P, _P, _I64, _I32, _I32, _P, _P, _I64, _P, _P, _P, _P, _I64, _P])
+_lib.register("stx_compact_rows", _I32, [_P, _P, _I64, _I32, _P, _P, _P, _P, _P, _P, _I64, _I32, _P])
+_lib.register("stx_coarse_rows", _I32, [_P, _P, _I64, _I32, _I32, _P, _P, _I64, _P, _P, _P, _P, _I64, _I32, _P])
 _lib.register("stx_prolong_set", _I32, [_P, _P, _I64, _P, _P, _I64, _I32, _P, _P, _P, _P, _P, _P])
 _lib.register("stx_prolong_rows", _I32, [_P, _I64, _I32, _P, _P])
 _lib.register("stx_fill_i32", _I32, [_P, _I64, _I32, _I32, _P])
@@ -129,12 +129,12 @@ def setop_raw(op: int, a, b, *, nrows: int, d_nrows: Optional[torch.Tensor] = No
     if g == 0 and (map_a is not None or map_b is not None or d_nrows is not None):
         g = choose_group(na + nb, nrows)
     nbytes = _lib.lib().stx_setop_scratch_bytes(nrows, g)
-    sp, sbytes = ctx.scratch(nbytes)
+    sp, sbytes, ep = ctx.scratch(nbytes)
     b_beg, b_end, b_off = b if b is not None else (None, None, None)
     check(_lib.lib().stx_setop(int(op), ptr(a_beg), ptr(a_end), ptr(a_off), ptr(map_a), int(xf_a[0]), int(xf_a[1]),
                                int(na), ptr(b_beg), ptr(b_end), ptr(b_off), ptr(map_b), int(xf_b[0]), int(xf_b[1]),
                                int(nb), ptr(d_nrows), int(nrows), int(g), ptr(out_off), ptr(ob), ptr(oe), cap,
-                               ctx.slot(0), ctx.status_ptr, sp, sbytes, ctx.stream), "setop")
+                               ctx.slot(0), ctx.status_ptr, sp, sbytes, ep, ctx.stream), "setop")
     return out_off, ob, oe, ctx
 
 
@@ -149,9 +149,9 @@ def rows_union(rows_a: torch.Tensor, shift_a: int, rows_b: torch.Tensor, shift_b
     map_b = empty_i32(na + nb, dev)
     tmp = empty_i32(2 * (nb + 1), dev)
     d_nu = empty_i32(1, dev)
-    sp, sbytes = ctx.scratch(_lib.lib().stx_scan_scratch_bytes(nb))
+    sp, sbytes, ep = ctx.scratch(_lib.lib().stx_scan_scratch_bytes(nb))
     check(_lib.lib().stx_rows_union(ptr(rows_a), na, int(shift_a), ptr(rows_b), nb, int(shift_b), ptr(rows_u),
-                                    ptr(map_a), ptr(map_b), ptr(d_nu), ptr(tmp), ctx.status_ptr, sp, sbytes, ctx.stream),
+                                    ptr(map_a), ptr(map_b), ptr(d_nu), ptr(tmp), ctx.status_ptr, sp, sbytes, ep, ctx.stream),
           "rows_union")
     return rows_u, map_a, map_b, d_nu
 
@@ -165,9 +165,9 @@ def compact_rows(rows: torch.Tensor, off: torch.Tensor, shift: int = 0):
     off_o = empty_i32(R + 1, dev)
     tmp = empty_i32(2 * (R + 1), dev)
     d_cnt = empty_i32(1, dev)
-    sp, sbytes = ctx.scratch(_lib.lib().stx_scan_scratch_bytes(R))
+    sp, sbytes, ep = ctx.scratch(_lib.lib().stx_scan_scratch_bytes(R))
     check(_lib.lib().stx_compact_rows(ptr(rows), ptr(off), R, int(shift), ptr(rows_o), ptr(off_o), ptr(d_cnt), ptr(tmp),
-                                      ctx.status_ptr, sp, sbytes, ctx.stream), "compact_rows")
+                                      ctx.status_ptr, sp, sbytes, ep, ctx.stream), "compact_rows")
     return rows_o, off_o, d_cnt
 
 
@@ -179,9 +179,9 @@ def coarse_rows(rows: torch.Tensor, off: torch.Tensor, ratio: int, all_rows: boo
     maps = empty_i32(ratio * R, dev) if want_maps else None
     tmp = empty_i32(2 * (R + 1), dev)
     d_cnt = empty_i32(1, dev)
-    sp, sbytes = ctx.scratch(_lib.lib().stx_scan_scratch_bytes(R))
+    sp, sbytes, ep = ctx.scratch(_lib.lib().stx_scan_scratch_bytes(R))
     check(_lib.lib().stx_coarse_rows(ptr(rows), ptr(off), R, int(ratio), int(bool(all_rows)), ptr(crow), ptr(maps), R,
-                                     ptr(d_cnt), ptr(tmp), ctx.status_ptr, sp, sbytes, ctx.stream), "coarse_rows")
+                                     ptr(d_cnt), ptr(tmp), ctx.status_ptr, sp, sbytes, ep, ctx.stream), "coarse_rows")
     return crow, maps, d_cnt
 
 

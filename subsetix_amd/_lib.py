@@ -29,12 +29,12 @@ _SIGNATURES: Dict[str, Tuple[object, list]] = {
     "stx_abi_version": (_I32, []),
     "stx_last_error": (ctypes.c_char_p, []),
     "stx_set_spin_limit": (None, [_I64]),
-    "stx_scan_i32": (_I32, [_P, _P, _I64, _P, _P, _P, _P, _I64, _P]),
+    "stx_scan_i32": (_I32, [_P, _P, _I64, _P, _P, _P, _P, _I64, _I32, _P]),
     "stx_scan_scratch_bytes": (_I64, [_I64]),
     "stx_setop_scratch_bytes": (_I64, [_I64, _I32]),
     "stx_setop": (_I32, [_I32, _P, _P, _P, _P, _I32, _I32, _I64, _P, _P, _P, _P, _I32, _I32, _I64, _P, _I64, _I32,
-                         _P, _P, _P, _I64, _P, _P, _P, _I64, _P]),
-    "stx_rows_union": (_I32, [_P, _I64, _I32, _P, _I64, _I32, _P, _P, _P, _P, _P, _P, _P, _I64, _P]),
+                         _P, _P, _P, _I64, _P, _P, _P, _I64, _I32, _P]),
+    "stx_rows_union": (_I32, [_P, _I64, _I32, _P, _I64, _I32, _P, _P, _P, _P, _P, _P, _P, _I64, _I32, _P]),
 }
 
 _lib = None
@@ -112,6 +112,7 @@ class Context:
         # [0] status bits, [1..] result slots
         self.words = torch.zeros(64, dtype=torch.int32, device=device)
         self._scratch = torch.zeros(1 << 16, dtype=torch.uint8, device=device)
+        self._epoch = 0
 
     @property
     def stream(self) -> int:
@@ -124,10 +125,16 @@ class Context:
     def slot(self, k: int) -> int:
         return self.words.data_ptr() + 4 * (1 + k)
 
-    def scratch(self, nbytes: int) -> Tuple[int, int]:
+    def scratch(self, nbytes: int) -> Tuple[int, int, int]:
+        """Look-back scratch of at least ``nbytes`` and a fresh launch epoch
+        (state words of earlier launches then read as unpublished)."""
         if nbytes > self._scratch.numel():
             self._scratch = torch.zeros(max(nbytes, 2 * self._scratch.numel()), dtype=torch.uint8, device=self.device)
-        return self._scratch.data_ptr(), self._scratch.numel()
+        self._epoch += 1
+        if self._epoch >= (1 << 30) - 1:
+            self._scratch.zero_()
+            self._epoch = 1
+        return self._scratch.data_ptr(), self._scratch.numel(), self._epoch
 
     def read_slots(self, n: int):
         """One D2H copy: [status, slot0, slot1, ...]; raises on a device flag."""

@@ -746,9 +746,9 @@ int32_t stx_interval_lengths(const int32_t* beg, const int32_t* end, int64_t n, 
 // width x height float field; CSR with dense rows.  Fused count/scan/write.
 int32_t stx_threshold(int32_t gradient, const float* data, int32_t width, int32_t height, float t, int32_t* out_off,
                       int32_t* out_beg, int32_t* out_end, int64_t capacity, int32_t* d_total, uint32_t* status,
-                      void* scratch, int64_t scratch_bytes, hipStream_t stream) {
+                      void* scratch, int64_t scratch_bytes, uint32_t epoch, hipStream_t stream) {
     if (width <= 0 || height <= 0) return stx_fail(STX_EINVAL, "threshold: width and height must be positive");
-    FusedLaunch L{nullptr, height, out_off, out_beg, out_end, capacity, d_total, status, scratch, scratch_bytes, stream};
+    FusedLaunch L{nullptr, height, out_off, out_beg, out_end, capacity, d_total, status, scratch, scratch_bytes, epoch, stream};
     if (gradient) return launch_fused_rows<64>(ThresholdRow<true>{data, width, height, t}, L, "threshold_gradient");
     return launch_fused_rows<64>(ThresholdRow<false>{data, width, height, t}, L, "threshold");
 }
@@ -757,9 +757,9 @@ int64_t stx_threshold_scratch_bytes(int32_t height) { return fused_rows_scratch_
 
 int32_t stx_value_runs(int32_t esize, int32_t group, const int32_t* off, int64_t R, const int32_t* beg, const int32_t* co,
                        const void* values, int64_t target_bits, int32_t* out_off, int32_t* out_beg, int32_t* out_end,
-                       int64_t capacity, int32_t* d_total, uint32_t* status, void* scratch, int64_t scratch_bytes,
+                       int64_t capacity, int32_t* d_total, uint32_t* status, void* scratch, int64_t scratch_bytes, uint32_t epoch,
                        hipStream_t stream) {
-    FusedLaunch L{nullptr, R, out_off, out_beg, out_end, capacity, d_total, status, scratch, scratch_bytes, stream};
+    FusedLaunch L{nullptr, R, out_off, out_beg, out_end, capacity, d_total, status, scratch, scratch_bytes, epoch, stream};
     if (esize == 1)
         return launch_fused_rows_g(group, ValueRunsRow<int8_t>{off, beg, co, (const int8_t*)values, (int8_t)target_bits},
                                    L, "value_runs");
@@ -773,8 +773,8 @@ int32_t stx_value_runs(int32_t esize, int32_t group, const int32_t* off, int64_t
 int32_t stx_grad_tag(int32_t dtype, int32_t group, const int32_t* rows, const int32_t* off, int64_t R, const int32_t* beg,
                      const int32_t* end, const int32_t* co, const void* u, double t, int32_t* out_off, int32_t* out_beg,
                      int32_t* out_end, int64_t capacity, int32_t* d_total, uint32_t* status, void* scratch,
-                     int64_t scratch_bytes, hipStream_t stream) {
-    FusedLaunch L{nullptr, R, out_off, out_beg, out_end, capacity, d_total, status, scratch, scratch_bytes, stream};
+                     int64_t scratch_bytes, uint32_t epoch, hipStream_t stream) {
+    FusedLaunch L{nullptr, R, out_off, out_beg, out_end, capacity, d_total, status, scratch, scratch_bytes, epoch, stream};
     if (dtype == 4)
         return launch_fused_rows_g(group, GradTagRow<float>{rows, off, beg, end, co, R, (const float*)u, (float)t}, L,
                                    "grad_tag");

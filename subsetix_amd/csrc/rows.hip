@@ -162,7 +162,8 @@ __global__ void add_kernel(const int32_t* __restrict__ src, int32_t* __restrict_
 }  // namespace
 
 extern "C" int32_t stx_scan_i32(const int32_t* in, const int32_t* d_n, int64_t n_bound, int32_t* out, int32_t* d_total,
-                                uint32_t* status, void* scratch, int64_t scratch_bytes, hipStream_t stream);
+                                uint32_t* status, void* scratch, int64_t scratch_bytes, uint32_t epoch,
+                                hipStream_t stream);
 
 extern "C" {
 
@@ -190,12 +191,12 @@ int32_t stx_rows_lookup(const int32_t* rows, int64_t R, int32_t y0, int64_t H, i
 // sized R / R+1; *d_count receives the kept row count.
 int32_t stx_compact_rows(const int32_t* rows, const int32_t* off, int64_t R, int32_t shift, int32_t* rows_out,
                          int32_t* off_out, int32_t* d_count, int32_t* tmp, uint32_t* status, void* scratch,
-                         int64_t scratch_bytes, hipStream_t stream) {
+                         int64_t scratch_bytes, uint32_t epoch, hipStream_t stream) {
     if (R < 0) return stx_fail(STX_EINVAL, "compact_rows: bad size");
     int32_t* flag = tmp;
     int32_t* pos = tmp + (R + 1);
     if (R > 0) hipLaunchKernelGGL(nonempty_flag_kernel, dim3(nblk(R)), dim3(kT), 0, stream, off, R, flag);
-    int32_t rc = stx_scan_i32(flag, nullptr, R, pos, d_count, status, scratch, scratch_bytes, stream);
+    int32_t rc = stx_scan_i32(flag, nullptr, R, pos, d_count, status, scratch, scratch_bytes, epoch, stream);
     if (rc) return rc;
     hipLaunchKernelGGL(compact_scatter_kernel, dim3(nblk(R + 1)), dim3(kT), 0, stream, rows, off, R, flag, pos, shift,
                        rows_out, off_out);
@@ -207,12 +208,12 @@ int32_t stx_compact_rows(const int32_t* rows, const int32_t* off, int64_t R, int
 // cap >= R.  *d_count receives the coarse row count.
 int32_t stx_coarse_rows(const int32_t* rows, const int32_t* off, int64_t R, int32_t ratio, int32_t all_rows,
                         int32_t* crow, int32_t* maps, int64_t cap, int32_t* d_count, int32_t* tmp, uint32_t* status,
-                        void* scratch, int64_t scratch_bytes, hipStream_t stream) {
+                        void* scratch, int64_t scratch_bytes, uint32_t epoch, hipStream_t stream) {
     if (R < 0 || ratio < 1 || cap < R) return stx_fail(STX_EINVAL, "coarse_rows: bad arguments");
     int32_t* flag = tmp;
     int32_t* pos = tmp + (R + 1);
     if (R > 0) hipLaunchKernelGGL(coarse_flag_kernel, dim3(nblk(R)), dim3(kT), 0, stream, rows, off, R, ratio, all_rows, flag);
-    int32_t rc = stx_scan_i32(flag, nullptr, R, pos, d_count, status, scratch, scratch_bytes, stream);
+    int32_t rc = stx_scan_i32(flag, nullptr, R, pos, d_count, status, scratch, scratch_bytes, epoch, stream);
     if (rc) return rc;
     if (R > 0)
         hipLaunchKernelGGL(coarse_scatter_kernel, dim3(nblk(R)), dim3(kT), 0, stream, rows, off, R, ratio, flag, pos, crow,

@@ -255,7 +255,7 @@ __global__ __launch_bounds__(kBlock) void setop_tile_kernel(Operand A, Operand B
                                                             int32_t* __restrict__ out_off, int32_t* __restrict__ ob,
                                                             int32_t* __restrict__ oe, int64_t capacity,
                                                             int32_t* __restrict__ d_total, uint32_t* status,
-                                                            void* scratch, int64_t spin_limit) {
+                                                            void* scratch, uint32_t epoch, int64_t spin_limit) {
     __shared__ int32_t s_kb[kPCAP];
     __shared__ int32_t s_ke[kPCAP];
     __shared__ int32_t s_ex[kPCAP + 1];
@@ -264,11 +264,10 @@ __global__ __launch_bounds__(kBlock) void setop_tile_kernel(Operand A, Operand B
     __shared__ int32_t s_M[kMaxRPT + 1];
     __shared__ int32_t s_scan[17];
     __shared__ SegMax s_seg[kWaves];
-    __shared__ uint32_t s_slot;
     __shared__ uint32_t s_excl;
 
     LookbackScratch lb = lookback_scratch(scratch, ntiles);
-    const uint32_t tile = lookback_ticket(lb, &s_slot);
+    const uint32_t tile = blockIdx.x;
     const int tid = threadIdx.x;
     const int64_t row0 = (int64_t)tile * RPT;
     const int nr = (int)(U - row0 < RPT ? (U - row0 > 0 ? U - row0 : 0) : RPT);
@@ -447,7 +446,7 @@ __global__ __launch_bounds__(kBlock) void setop_tile_kernel(Operand A, Operand B
         if (tid == 0) s_ex[nr] = tile_total;
     }
     if (wave_id() == 0) {
-        uint32_t e = lookback_exclusive(lb, tile, (uint32_t)tile_total, status, spin_limit);
+        uint32_t e = lookback_exclusive(lb, tile, (uint32_t)tile_total, epoch, status, spin_limit);
         if (lane_id() == 0) s_excl = e;
     }
     __syncthreads();
@@ -505,7 +504,6 @@ __global__ __launch_bounds__(kBlock) void setop_tile_kernel(Operand A, Operand B
             setop_row<64, OP, true>(A, a0, na, B, b0, nb, ob, oe, excl + s_ex[r]);
         }
     }
-    lookback_finish(lb, ntiles, &s_slot);
 }
 
 // ---------------------------------------------------------------------------
@@ -517,12 +515,11 @@ constexpr int kScanTile = kBlock * kScanItems;
 __global__ __launch_bounds__(kBlock) void scan_i32_kernel(const int32_t* __restrict__ in, const int32_t* d_n, int64_t n_bound,
                                                           uint32_t ntiles, int32_t* __restrict__ out,
                                                           int32_t* __restrict__ d_total, uint32_t* status, void* scratch,
-                                                          int64_t spin_limit) {
+                                                          uint32_t epoch, int64_t spin_limit) {
     __shared__ int32_t s_scan[17];
-    __shared__ uint32_t s_slot;
     __shared__ uint32_t s_excl;
     LookbackScratch lb = lookback_scratch(scratch, ntiles);
-    const uint32_t tile = lookback_ticket(lb, &s_slot);
+    const uint32_t tile = blockIdx.x;
     const int64_t n = d_n ? (int64_t)*d_n : n_bound;
     const int64_t t0 = (int64_t)tile * kScanTile + (int64_t)threadIdx.x * kScanItems;
     int32_t vals[kScanItems];
@@ -536,7 +533,7 @@ __global__ __launch_bounds__(kBlock) void scan_i32_kernel(const int32_t* __restr
     int32_t total = 0;
     int32_t ex = block_exclusive_scan<int32_t>(sum, s_scan, &total);
     if (wave_id() == 0) {
-        uint32_t e = lookback_exclusive(lb, tile, (uint32_t)total, status, spin_limit);
+        uint32_t e = lookback_exclusive(lb, tile, (uint32_t)total, epoch, status, spin_limit);
         if (lane_id() == 0) s_excl = e;
     }
     __syncthreads();
@@ -552,7 +549,6 @@ __global__ __launch_bounds__(kBlock) void scan_i32_kernel(const int32_t* __restr
         out[n] = run;
         if (d_total) *d_total = run;
     }
-    lookback_finish(lb, ntiles, &s_slot);
 }
 
 // ---------------------------------------------------------------------------
@@ -634,8 +630,8 @@ static int32_t setop_tile_rows(int64_t elems, int64_t rows) {
 template <int G>
 static int32_t launch_setop_g(int32_t op, const Operand& A, const Operand& B, const int32_t* d_nrows, int64_t nrows_bound,
                               int32_t* out_off, int32_t* ob, int32_t* oe, int64_t capacity, int32_t* d_total,
-                              uint32_t* status, void* scratch, int64_t scratch_bytes, hipStream_t stream) {
-    FusedLaunch L{d_nrows, nrows_bound, out_off, ob, oe, capacity, d_total, status, scratch, scratch_bytes, stream};
+                              uint32_t* status, void* scratch, int64_t scratch_bytes, uint32_t epoch, hipStream_t stream) {
+    FusedLaunch L{d_nrows, nrows_bound, out_off, ob, oe, capacity, d_total, status, scratch, scratch_bytes, epoch, stream};
     if (op == 0) return launch_fused_rows<G>(SetopRow<0>{A, B}, L, "setop_union");
     if (op == 1) return launch_fused_rows<G>(SetopRow<1>{A, B}, L, "setop_intersection");
     return launch_fused_rows<G>(SetopRow<2>{A, B}, L, "setop_difference");
@@ -644,13 +640,13 @@ static int32_t launch_setop_g(int32_t op, const Operand& A, const Operand& B, co
 extern "C" {
 
 int32_t stx_scan_i32(const int32_t* in, const int32_t* d_n, int64_t n_bound, int32_t* out, int32_t* d_total,
-                     uint32_t* status, void* scratch, int64_t scratch_bytes, hipStream_t stream) {
+                     uint32_t* status, void* scratch, int64_t scratch_bytes, uint32_t epoch, hipStream_t stream) {
     if (n_bound < 0) return STX_EINVAL;
     uint32_t ntiles = (uint32_t)((n_bound + 1 + kScanTile - 1) / kScanTile);
     if (ntiles == 0) ntiles = 1;
     if (lookback_bytes(ntiles) > scratch_bytes) return stx_fail(STX_EINVAL, "scan: scratch too small");
     hipLaunchKernelGGL(scan_i32_kernel, dim3(ntiles), dim3(kBlock), 0, stream, in, d_n, n_bound, ntiles, out, d_total,
-                       status, scratch, stx_spin_limit());
+                       status, scratch, epoch, stx_spin_limit());
     return stx_launch_status("scan_i32_kernel");
 }
 
@@ -672,7 +668,7 @@ int32_t stx_setop(int32_t op, const int32_t* a_beg, const int32_t* a_end, const 
                   const int32_t* b_off, const int32_t* b_map, int32_t b_xf_mode, int32_t b_xf_param, int64_t b_n,
                   const int32_t* d_nrows, int64_t nrows_bound, int32_t group, int32_t* out_off, int32_t* out_beg,
                   int32_t* out_end, int64_t capacity, int32_t* d_total, uint32_t* status, void* scratch,
-                  int64_t scratch_bytes, hipStream_t stream) {
+                  int64_t scratch_bytes, uint32_t epoch, hipStream_t stream) {
     if (op < 0 || op > 2 || nrows_bound < 0 || a_off == nullptr) return stx_fail(STX_EINVAL, "setop: bad arguments");
     if (op != 0 && b_off == nullptr) return stx_fail(STX_EINVAL, "setop: intersection/difference need two operands");
     Operand A{a_beg, a_end, a_off, a_map, XForm{a_xf_mode, a_xf_param}};
@@ -687,29 +683,29 @@ int32_t stx_setop(int32_t op, const int32_t* a_beg, const int32_t* a_end, const 
         const int64_t spin = stx_spin_limit();
         if (op == 0)
             hipLaunchKernelGGL(setop_tile_kernel<0>, dim3(ntiles), dim3(kBlock), 0, stream, A, B, nrows_bound, rpt, ntiles,
-                               out_off, out_beg, out_end, capacity, d_total, status, scratch, spin);
+                               out_off, out_beg, out_end, capacity, d_total, status, scratch, epoch, spin);
         else if (op == 1)
             hipLaunchKernelGGL(setop_tile_kernel<1>, dim3(ntiles), dim3(kBlock), 0, stream, A, B, nrows_bound, rpt, ntiles,
-                               out_off, out_beg, out_end, capacity, d_total, status, scratch, spin);
+                               out_off, out_beg, out_end, capacity, d_total, status, scratch, epoch, spin);
         else
             hipLaunchKernelGGL(setop_tile_kernel<2>, dim3(ntiles), dim3(kBlock), 0, stream, A, B, nrows_bound, rpt, ntiles,
-                               out_off, out_beg, out_end, capacity, d_total, status, scratch, spin);
+                               out_off, out_beg, out_end, capacity, d_total, status, scratch, epoch, spin);
         return stx_launch_status("setop_tile_kernel");
     }
     const int32_t g = group > 0 ? group : 64;
     if (g >= 64)
         return launch_setop_g<64>(op, A, B, d_nrows, nrows_bound, out_off, out_beg, out_end, capacity, d_total, status,
-                                  scratch, scratch_bytes, stream);
+                                  scratch, scratch_bytes, epoch, stream);
     if (g >= 16)
         return launch_setop_g<16>(op, A, B, d_nrows, nrows_bound, out_off, out_beg, out_end, capacity, d_total, status,
-                                  scratch, scratch_bytes, stream);
+                                  scratch, scratch_bytes, epoch, stream);
     return launch_setop_g<4>(op, A, B, d_nrows, nrows_bound, out_off, out_beg, out_end, capacity, d_total, status,
-                             scratch, scratch_bytes, stream);
+                             scratch, scratch_bytes, epoch, stream);
 }
 
 int32_t stx_rows_union(const int32_t* rows_a, int64_t na, int32_t shift_a, const int32_t* rows_b, int64_t nb,
                        int32_t shift_b, int32_t* rows_u, int32_t* map_a, int32_t* map_b, int32_t* d_nu, int32_t* tmp,
-                       uint32_t* status, void* scratch, int64_t scratch_bytes, hipStream_t stream) {
+                       uint32_t* status, void* scratch, int64_t scratch_bytes, uint32_t epoch, hipStream_t stream) {
     // tmp: 2*(nb+1) int32 (flags, positions)
     if (na < 0 || nb < 0) return stx_fail(STX_EINVAL, "rows_union: bad sizes");
     int32_t* flag = tmp;
@@ -717,7 +713,7 @@ int32_t stx_rows_union(const int32_t* rows_a, int64_t na, int32_t shift_a, const
     if (nb > 0)
         hipLaunchKernelGGL(rows_newflag_kernel, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, stream, rows_a, na, shift_a,
                            rows_b, nb, shift_b, flag);
-    int32_t rc = stx_scan_i32(flag, nullptr, nb, pos, nullptr, status, scratch, scratch_bytes, stream);
+    int32_t rc = stx_scan_i32(flag, nullptr, nb, pos, nullptr, status, scratch, scratch_bytes, epoch, stream);
     if (rc) return rc;
     if (na + nb > 0)
         hipLaunchKernelGGL(rows_scatter_kernel, dim3((unsigned)((na + nb + 255) / 256)), dim3(256), 0, stream, rows_a, na,

@@ -109,20 +109,22 @@ __device__ __forceinline__ void xform_apply(const XForm& xf, int32_t& b, int32_t
 
 // ---- decoupled look-back ---------------------------------------------------
 // Scratch layout shared by every single-pass kernel (caller-provided, zeroed
-// once at allocation, left zeroed by the last block of every launch):
-//   [0] ticket counter   [1] done counter   [2..] per-tile state words.
-// A state word packs (flag << 32) | value with flag 1 = aggregate, 2 = prefix.
+// once at allocation): a 64-byte header (unused) then one 64-bit state word
+// per tile.  A word packs (epoch << 34) | (flag << 32) | value, flag 1 =
+// aggregate, 2 = inclusive prefix.  Every launch passes a fresh epoch (the
+// host counts launches per scratch buffer), so words left by earlier launches
+// read as "not published" and nothing has to be cleared between launches.
+// Tiles are numbered by blockIdx: workgroups are dispatched in increasing id
+// order, so every tile a block waits on belongs to a block that has already
+// been dispatched; the wait is also bounded (spin limit), so a block always
+// exits.
 struct LookbackScratch {
-    uint32_t* ticket;
-    uint32_t* done;
     unsigned long long* state;
     int64_t max_tiles;
 };
 
 __device__ __forceinline__ LookbackScratch lookback_scratch(void* base, int64_t max_tiles) {
     LookbackScratch s;
-    s.ticket = (uint32_t*)base;
-    s.done = (uint32_t*)base + 1;
     s.state = (unsigned long long*)((char*)base + 64);
     s.max_tiles = max_tiles;
     return s;
@@ -132,8 +134,8 @@ __device__ __forceinline__ LookbackScratch lookback_scratch(void* base, int64_t 
 // with it: relaxed device-scope atomics are enough.  (Acquire/release at
 // device scope would write back / invalidate the whole L2 of the XCD on every
 // poll, because the L2s of the XCDs are not coherent with each other.)
-__device__ __forceinline__ void state_store(unsigned long long* p, uint32_t flag, uint32_t value) {
-    unsigned long long w = ((unsigned long long)flag << 32) | (unsigned long long)value;
+__device__ __forceinline__ void state_store(unsigned long long* p, uint32_t epoch, uint32_t flag, uint32_t value) {
+    unsigned long long w = ((unsigned long long)((epoch << 2) | flag) << 32) | (unsigned long long)value;
     __hip_atomic_store(p, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
@@ -142,26 +144,29 @@ __device__ __forceinline__ unsigned long long state_load(const unsigned long lon
 }
 
 // Called by all 64 lanes of ONE wave of the block.  Publishes `agg` for tile
-// `tile` and returns the exclusive prefix of all earlier tiles.  The wait is
-// bounded: after `spin_limit` polls the block records STX_FLAG_SPIN in
-// `status` and proceeds with what it has (its result is then invalid, but
-// every wave still exits).
-__device__ __forceinline__ uint32_t lookback_exclusive(LookbackScratch s, uint32_t tile, uint32_t agg,
+// `tile` and returns the exclusive prefix of all earlier tiles.  After
+// `spin_limit` polls the block records STX_FLAG_SPIN in `status` and proceeds
+// (its result is then invalid, but every wave still exits).
+__device__ __forceinline__ uint32_t lookback_exclusive(LookbackScratch s, uint32_t tile, uint32_t agg, uint32_t epoch,
                                                        uint32_t* status, int64_t spin_limit) {
     const int lane = lane_id();
     if (tile == 0) {
-        if (lane == 0) state_store(&s.state[0], 2u, agg);
+        if (lane == 0) state_store(&s.state[0], epoch, 2u, agg);
         return 0u;
     }
-    if (lane == 0) state_store(&s.state[tile], 1u, agg);
+    if (lane == 0) state_store(&s.state[tile], epoch, 1u, agg);
     uint32_t excl = 0;
     int64_t pos = (int64_t)tile - 1;  // nearest predecessor not yet folded in
     int64_t spins = 0;
     while (pos >= 0) {
         int64_t p = pos - lane;
-        unsigned long long w = p >= 0 ? state_load(&s.state[p]) : (2ull << 32);
-        uint32_t flag = (uint32_t)(w >> 32);
-        uint32_t val = (uint32_t)w;
+        uint32_t flag = 2u, val = 0u;
+        if (p >= 0) {
+            unsigned long long w = state_load(&s.state[p]);
+            uint32_t hi = (uint32_t)(w >> 32);
+            flag = (hi >> 2) == epoch ? (hi & 3u) : 0u;
+            val = (uint32_t)w;
+        }
         uint64_t not_ready = __ballot(flag == 0u);
         uint64_t is_prefix = __ballot(flag == 2u);
         // lanes up to (and including) the first prefix lane are usable only if
@@ -177,44 +182,14 @@ __device__ __forceinline__ uint32_t lookback_exclusive(LookbackScratch s, uint32
             continue;
         }
         uint32_t contrib = (lane <= first_prefix && p >= 0) ? val : 0u;
-        // wave sum
 #pragma unroll
         for (int d = 32; d >= 1; d >>= 1) contrib += __shfl_xor(contrib, d, 64);
         excl += contrib;
         if (first_prefix < 64) break;
         pos -= 64;
     }
-    if (lane == 0) state_store(&s.state[tile], 2u, excl + agg);
+    if (lane == 0) state_store(&s.state[tile], epoch, 2u, excl + agg);
     return excl;
-}
-
-// Dynamic tile id: tiles are numbered in the order blocks start, so every
-// tile a block waits on belongs to a block that is already running.
-__device__ __forceinline__ uint32_t lookback_ticket(LookbackScratch s, uint32_t* lds_slot) {
-    __syncthreads();
-    if (threadIdx.x == 0) *lds_slot = atomicAdd(s.ticket, 1u);
-    __syncthreads();
-    return *lds_slot;
-}
-
-// Called by ALL threads at the very end of a block: the last block to finish
-// clears the scratch (in parallel) for the next launch.
-__device__ __forceinline__ void lookback_finish(LookbackScratch s, uint32_t ntiles, uint32_t* lds_slot) {
-    // Every state read of this block has returned before the increment (the
-    // values were consumed), so the block that counts last may clear the
-    // words; the next launch sees the zeros at the kernel boundary.
-    __syncthreads();
-    if (threadIdx.x == 0) *lds_slot = atomicAdd(s.done, 1u) == ntiles - 1u ? 1u : 0u;
-    __syncthreads();
-    if (*lds_slot) {
-        for (uint32_t t = threadIdx.x; t < ntiles; t += blockDim.x)
-            __hip_atomic_store(&s.state[t], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            __hip_atomic_store(s.ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(s.done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-    }
 }
 
 // Bytes of look-back scratch needed for `ntiles` tiles.

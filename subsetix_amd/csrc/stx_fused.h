@@ -35,16 +35,15 @@ __global__ __launch_bounds__(kBlock) void fused_rows_kernel(RowOp op, const int3
                                                             int32_t* __restrict__ out_off, int32_t* __restrict__ ob,
                                                             int32_t* __restrict__ oe, int64_t capacity,
                                                             int32_t* __restrict__ d_total, uint32_t* status,
-                                                            void* scratch, int64_t spin_limit) {
+                                                            void* scratch, uint32_t epoch, int64_t spin_limit) {
     constexpr int RPT = TileShape<G>::rows;
     __shared__ int32_t s_cnt[RPT];
     __shared__ int32_t s_pref[RPT];
     __shared__ int32_t s_scan[17];
-    __shared__ uint32_t s_slot;
     __shared__ uint32_t s_excl;
 
     LookbackScratch lb = lookback_scratch(scratch, ntiles);
-    const uint32_t tile = lookback_ticket(lb, &s_slot);
+    const uint32_t tile = blockIdx.x;
     const int64_t U = d_nrows ? (int64_t)*d_nrows : nrows_bound;
     const int64_t row0 = (int64_t)tile * RPT;
     const int wave = wave_id();
@@ -64,7 +63,7 @@ __global__ __launch_bounds__(kBlock) void fused_rows_kernel(RowOp op, const int3
     int32_t ex = block_exclusive_scan<int32_t>(v, s_scan, &tile_total);
     if ((int)threadIdx.x < RPT) s_pref[threadIdx.x] = ex;
     if (wave == 0) {
-        uint32_t e = lookback_exclusive(lb, tile, (uint32_t)tile_total, status, spin_limit);
+        uint32_t e = lookback_exclusive(lb, tile, (uint32_t)tile_total, epoch, status, spin_limit);
         if (lane_id() == 0) s_excl = e;
     }
     __syncthreads();
@@ -87,7 +86,6 @@ __global__ __launch_bounds__(kBlock) void fused_rows_kernel(RowOp op, const int3
             if (u < U && s_cnt[local] > 0) op.template run<G, true>(u, ob, oe, excl + s_pref[local]);
         }
     }
-    lookback_finish(lb, ntiles, &s_slot);
 }
 
 struct FusedLaunch {
@@ -101,6 +99,7 @@ struct FusedLaunch {
     uint32_t* status;
     void* scratch;
     int64_t scratch_bytes;
+    uint32_t epoch;
     hipStream_t stream;
 };
 
@@ -123,7 +122,7 @@ inline int32_t launch_fused_rows(const RowOp& op, const FusedLaunch& L, const ch
     if (lookback_bytes(ntiles) > L.scratch_bytes) return stx_fail(STX_EINVAL, "fused rows: scratch too small");
     hipLaunchKernelGGL((fused_rows_kernel<G, RowOp>), dim3(ntiles), dim3(kBlock), 0, L.stream, op, L.d_nrows,
                        L.nrows_bound, ntiles, L.out_off, L.ob, L.oe, L.capacity, L.d_total, L.status, L.scratch,
-                       stx_spin_limit());
+                       L.epoch, stx_spin_limit());
     return stx_launch_status(what);
 }
 

@@ -1,0 +1,105 @@
+"""Level-to-level kernels on interval fields: fused transfers onto a target
+subset, gradient tagging on sparse levels and the batched upwind step.
+
+These are the building blocks of the sparse multi-level drivers
+(``subsetix_amd.amr2.composite`` and ``subsetix_amd.batch``): unlike the
+reference's synchronisation (restrict the whole fine field, gather, scatter;
+fields.py:570-612) each transfer is one pass over the target cells.
+"""
+
+from __future__ import annotations
+
+import torch
+
+from . import _core, _lib
+from ._lib import _F64, _I32, _I64, _P, check, context, ptr
+from .expressions import IntervalSet
+from .interval_field import IntervalField, cell_offsets
+
+_lib.register("stx_transfer", _I32, [_I32, _I32, _I32, _I32, _I32, _P, _P, _P, _P, _P, _I64, _P, _P, _P, _P, _P, _I64,
+                                     _P, _P, _P, _P, _P, _I64, _P, _P, _P])
+_lib.register("stx_grad_tag", _I32, [_I32, _I32, _P, _P, _I64, _P, _P, _P, _P, _F64, _P, _P, _P, _I64, _P, _P, _P,
+                                     _I64, _I32, _P])
+_lib.register("stx_upwind_batched", _I32, [_I32, _I32, _P, _P, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P])
+
+
+def _geom(s: IntervalSet, co: torch.Tensor):
+    return (ptr(s.rows), ptr(s.row_offsets), ptr(s.begin), ptr(s.end), ptr(co), s.row_count)
+
+
+def _group(cells: int, rows: int) -> int:
+    avg = cells / max(rows, 1)
+    return 64 if avg >= 96 else 16 if avg >= 12 else 4
+
+
+def _dtype_code(t: torch.Tensor) -> int:
+    if t.dtype == torch.float32:
+        return 4
+    if t.dtype == torch.float64:
+        return 8
+    raise TypeError("transfers need float32 or float64 fields")
+
+
+def prolong_into(target: IntervalSet, dst: IntervalField, src: IntervalField, ratio: int) -> None:
+    """dst(y, x) = src(y // ratio, x // ratio) for every cell of ``target``
+    (cells of target must lie in dst's geometry)."""
+    if target.interval_count == 0:
+        return
+    if dst.values.dtype != src.values.dtype:
+        raise TypeError("source and destination fields must share a dtype")
+    ctx = context(dst.values.device)
+    tco = cell_offsets(target)
+    n = int(dst.values.numel())
+    check(_lib.lib().stx_transfer(0, _dtype_code(dst.values), 0, _group(n, target.row_count), int(ratio),
+                                  *_geom(target, tco), *_geom(dst.interval_set, dst.interval_cell_offsets),
+                                  *_geom(src.interval_set, src.interval_cell_offsets), ptr(src.values),
+                                  ptr(dst.values), ctx.stream), "prolong_into")
+
+
+def restrict_into(target: IntervalSet, dst: IntervalField, src: IntervalField, ratio: int, *,
+                  accumulate: str = "f64") -> None:
+    """dst(y, x) = mean of src(y*R + dy, x*R + dx) over dy, dx < R (children
+    in dy-major order) for every cell of ``target``."""
+    if target.interval_count == 0:
+        return
+    if dst.values.dtype != src.values.dtype:
+        raise TypeError("source and destination fields must share a dtype")
+    ctx = context(dst.values.device)
+    tco = cell_offsets(target)
+    acc64 = 1 if accumulate == "f64" else 0
+    check(_lib.lib().stx_transfer(1, _dtype_code(dst.values), acc64, _group(int(src.values.numel()), target.row_count),
+                                  int(ratio), *_geom(target, tco), *_geom(dst.interval_set, dst.interval_cell_offsets),
+                                  *_geom(src.interval_set, src.interval_cell_offsets), ptr(src.values),
+                                  ptr(dst.values), ctx.stream), "restrict_into")
+
+
+def gradient_tag(field: IntervalField, threshold: float) -> IntervalSet:
+    """Runs of cells with |grad u| >= threshold (central differences where
+    both neighbours are in the set, one-sided otherwise), rows of the field."""
+    s = field.interval_set
+    dev = field.values.device
+    ctx = context(dev)
+    n = int(field.values.numel())
+    cap = max(n // 2 + s.interval_count + 1, 1)
+    out_off = _core.empty_i32(s.row_count + 1, dev)
+    ob = _core.empty_i32(cap, dev)
+    oe = _core.empty_i32(cap, dev)
+    g = _group(n, s.row_count)
+    sp, sbytes, ep = ctx.scratch(_lib.lib().stx_setop_scratch_bytes(s.row_count, 4))
+    check(_lib.lib().stx_grad_tag(_dtype_code(field.values), g, ptr(s.rows), ptr(s.row_offsets), s.row_count,
+                                  ptr(s.begin), ptr(s.end), ptr(field.interval_cell_offsets), ptr(field.values), float(threshold), ptr(out_off), ptr(ob), ptr(oe), cap, ctx.slot(0),
+                                  ctx.status_ptr, sp, sbytes, ep, ctx.stream), "grad_tag")
+    (total,) = _core.read_counts(ctx, ctx.words[1])
+    return IntervalSet._trusted(ob[:total], oe[:total], out_off, s.rows, dense=s.is_dense_rows())
+
+
+def upwind_batched(field: IntervalField, values: torch.Tensor, out: torch.Tensor, row_inst: torch.Tensor,
+                   inst_w: torch.Tensor, inst_coef: torch.Tensor) -> torch.Tensor:
+    """One upwind step over a stack of instances (see stx_upwind_batched)."""
+    s = field.interval_set
+    ctx = context(values.device)
+    check(_lib.lib().stx_upwind_batched(_dtype_code(values), _group(int(values.numel()), s.row_count), ptr(s.rows),
+                                        ptr(s.row_offsets), s.row_count, ptr(s.begin), ptr(s.end),
+                                        ptr(field.interval_cell_offsets), ptr(values), ptr(out), ptr(row_inst),
+                                        ptr(inst_w), ptr(inst_coef), ctx.stream), "upwind_batched")
+    return out

@@ -1,0 +1,195 @@
+"""GPU parity: fused level transfers, sparse gradient tagging, the sparse
+two-level composite step (closed forms / oracle restatements)."""
+
+import numpy as np
+import pytest
+
+from oracle import oracle as ox
+from tests.golden_util import assert_set_equal, canonical_rows, random_canonical, runs_of
+from tests.gpu_util import to_device, to_host
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def sx():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    import subsetix_amd
+
+    return subsetix_amd
+
+
+def _cells(s, vals):
+    co = ox.cell_offsets(s)
+    ir = ox.interval_rows(s)
+    d = {}
+    for i in range(s.n):
+        for k in range(int(s.end[i] - s.beg[i])):
+            d[(int(ir[i]), int(s.beg[i]) + k)] = vals[co[i] + k]
+    return d
+
+
+def _field(sx, s, vals, dtype):
+    import torch
+
+    f = sx.create_interval_field(to_device(s), 0.0, dtype=dtype)
+    f.values.copy_(torch.from_numpy(np.asarray(vals)).to(dtype))
+    return f
+
+
+@pytest.mark.parametrize("dt", ["f32", "f64"])
+def test_prolong_restrict_into(sx, dt):
+    import torch
+
+    from subsetix_amd.levels import prolong_into, restrict_into
+
+    tdt = torch.float32 if dt == "f32" else torch.float64
+    npdt = np.float32 if dt == "f32" else np.float64
+    rng = np.random.default_rng(1)
+    coarse = random_canonical(rng, 30, 60, mean_n=3, rows=np.arange(30))
+    fine = random_canonical(rng, 60, 120, mean_n=5, rows=np.arange(60))
+    target_f = random_canonical(rng, 60, 120, mean_n=4, rows=np.arange(60))
+    target_f = ox.intersection(target_f, fine)
+    cv = rng.random(int(ox.cell_offsets(coarse)[-1])).astype(npdt)
+    fv = rng.random(int(ox.cell_offsets(fine)[-1])).astype(npdt)
+    cf, ff = _field(sx, coarse, cv, tdt), _field(sx, fine, fv, tdt)
+    prolong_into(to_device(target_f), ff, cf, 2)
+    cc = _cells(coarse, cv)
+    exp = fv.copy()
+    co = ox.cell_offsets(fine)
+    ir = ox.interval_rows(fine)
+    tcells = set(ox.cells(target_f))
+    for i in range(fine.n):
+        for k in range(int(fine.end[i] - fine.beg[i])):
+            y, x = int(ir[i]), int(fine.beg[i]) + k
+            if (y, x) in tcells and (y // 2, x // 2) in cc:
+                exp[co[i] + k] = cc[(y // 2, x // 2)]
+    np.testing.assert_array_equal(ff.values.cpu().numpy(), exp)
+    # restrict onto a coarse target
+    target_c = ox.intersection(random_canonical(rng, 30, 60, mean_n=3, rows=np.arange(30)), coarse)
+    fv2 = rng.random(fv.size).astype(npdt)
+    ff.values.copy_(torch.from_numpy(fv2))
+    cf2 = _field(sx, coarse, cv, tdt)
+    restrict_into(to_device(target_c), cf2, ff, 2, accumulate="f64")
+    fc = _cells(fine, fv2)
+    expc = cv.copy()
+    cco = ox.cell_offsets(coarse)
+    cir = ox.interval_rows(coarse)
+    tc = set(ox.cells(target_c))
+    for i in range(coarse.n):
+        for k in range(int(coarse.end[i] - coarse.beg[i])):
+            y, x = int(cir[i]), int(coarse.beg[i]) + k
+            if (y, x) in tc:
+                acc = 0.0
+                for dy in range(2):
+                    for dx in range(2):
+                        acc += float(fc.get((2 * y + dy, 2 * x + dx), 0.0))
+                expc[cco[i] + k] = npdt(acc * 0.25)
+    np.testing.assert_array_equal(cf2.values.cpu().numpy(), expc)
+
+
+def _grad_tag_model(s, vals, t):
+    cells = _cells(s, vals)
+    tagged = set()
+    for (y, x), c in cells.items():
+        l, r, d, u = (y, x - 1), (y, x + 1), (y - 1, x), (y + 1, x)
+        gx = gy = 0.0
+        if l in cells and r in cells:
+            gx = (cells[r] - cells[l]) * 0.5
+        elif r in cells:
+            gx = cells[r] - c
+        elif l in cells:
+            gx = c - cells[l]
+        if d in cells and u in cells:
+            gy = (cells[u] - cells[d]) * 0.5
+        elif u in cells:
+            gy = cells[u] - c
+        elif d in cells:
+            gy = c - cells[d]
+        gx, gy = np.float32(gx), np.float32(gy)
+        if np.sqrt(np.float32(gx * gx) + np.float32(gy * gy)) >= np.float32(t):
+            tagged.add((y, x))
+    return tagged
+
+
+def test_gradient_tag_sparse_and_dense(sx):
+    import torch
+
+    from subsetix_amd.levels import gradient_tag
+
+    rng = np.random.default_rng(9)
+    s = random_canonical(rng, 40, 80, mean_n=4, rows=np.arange(40), touching=True)
+    v = rng.random(int(ox.cell_offsets(s)[-1])).astype(np.float32)
+    got = to_host(gradient_tag(_field(sx, s, v, torch.float32), 0.3))
+    assert ox.cells(got) == _grad_tag_model(s, v, 0.3)
+    # runs are maximal within adjacency
+    for row, ivs in canonical_rows(got).items():
+        for (b0, e0), (b1, e1) in zip(ivs, ivs[1:]):
+            assert e0 < b1
+    # dense: identical to the dense gradient tag of the reference formula
+    w, h = 70, 33
+    d = rng.random((h, w)).astype(np.float32)
+    dense = ox.full(w, h)
+    got = to_host(gradient_tag(_field(sx, dense, d.reshape(-1), torch.float32), 0.4))
+    assert_set_equal(got, ox.gradient_tag(d, w, h, 0.4))
+
+
+def _oracle_composite_step(c_set, cv, ext, fv, ghost, refine, W, H, vel, dt, dtype):
+    """prolong ghosts, upwind both levels, restrict (FP64 sums) - on dicts."""
+    npdt = dtype
+    cc = _cells(c_set, cv)
+    fco = ox.cell_offsets(ext)
+    fir = ox.interval_rows(ext)
+    gcells = ox.cells(ghost)
+    fv = fv.copy()
+    for i in range(ext.n):
+        for k in range(int(ext.end[i] - ext.beg[i])):
+            y, x = int(fir[i]), int(ext.beg[i]) + k
+            if (y, x) in gcells:
+                fv[fco[i] + k] = cc[(y // 2, x // 2)]
+    cv2 = ox.upwind(c_set, cv, width=W, height=H, a=vel[0], b=vel[1], dt=dt, dx=1.0 / W, dy=1.0 / H)
+    fv2 = ox.upwind(ext, fv, width=2 * W, height=2 * H, a=vel[0], b=vel[1], dt=dt, dx=0.5 / W, dy=0.5 / H)
+    fc = _cells(ext, fv2)
+    cco = ox.cell_offsets(c_set)
+    cir = ox.interval_rows(c_set)
+    rc = ox.cells(refine)
+    for i in range(c_set.n):
+        for k in range(int(c_set.end[i] - c_set.beg[i])):
+            y, x = int(cir[i]), int(c_set.beg[i]) + k
+            if (y, x) in rc:
+                acc = 0.0
+                for dy in range(2):
+                    for dx in range(2):
+                        acc += float(fc.get((2 * y + dy, 2 * x + dx), 0.0))
+                cv2[cco[i] + k] = npdt(acc * 0.25)
+    return cv2, fv2
+
+
+@pytest.mark.parametrize("dt", ["f32", "f64"])
+def test_composite_steps_vs_oracle(sx, dt):
+    import torch
+
+    from subsetix_amd.amr2.composite import CompositeTwoLevel
+
+    tdt = torch.float32 if dt == "f32" else torch.float64
+    npdt = np.float32 if dt == "f32" else np.float64
+    W = H = 24
+    refine = ox.make([0] * 6 + list(range(1, 11)) + [10] * 9, [5, 4, 4, 5, 6, 6, 7, 8, 9, 9],
+                     [12, 13, 14, 14, 15, 16, 16, 15, 14, 13])
+    comp = CompositeTwoLevel(W, H, to_device(refine), dtype=tdt, velocity=(0.8, -0.6), cfl=0.5)
+    rng = np.random.default_rng(2)
+    cv = rng.random(W * H).astype(npdt)
+    fv = rng.random(comp.fine_field.values.numel()).astype(npdt)
+    comp.set_values(torch.from_numpy(cv), torch.from_numpy(fv))
+    c_set, ext, ghost, ref = ox.full(W, H), to_host(comp.ext), to_host(comp.ghost), to_host(comp.refine)
+    hc = comp.coarse.values.cpu().numpy()
+    hf = comp.fine_field.values.cpu().numpy()
+    for _ in range(5):
+        comp.step()
+        hc, hf = _oracle_composite_step(c_set, hc, ext, hf, ghost, ref, W, H, (0.8, -0.6), comp.dt, npdt)
+    np.testing.assert_array_equal(comp.coarse.values.cpu().numpy(), hc)
+    np.testing.assert_array_equal(comp.fine_field.values.cpu().numpy(), hf)
+    assert abs(comp.mass() - ox.mass(hc, 1.0 / W / H)) < 1e-12
