@@ -196,6 +196,7 @@ class BatchedAMR:
         for t in self.ghosts[1:] + self.refines:
             cell_offsets(t)
         self.graph = None
+        self.cells = self.cells_per_step()
         torch.cuda.synchronize(self.dev)
         self.timings["regrid_s"] = self.timings.get("regrid_s", 0.0) + time.perf_counter() - t0
 

@@ -272,8 +272,10 @@ __global__ __launch_bounds__(kBlock) void setop_tile_kernel(Operand A, Operand B
     const int64_t row0 = (int64_t)tile * RPT;
     const int nr = (int)(U - row0 < RPT ? (U - row0 > 0 ? U - row0 : 0) : RPT);
 
-    const int32_t a_base = nr > 0 ? A.off[row0] : 0;
-    const int32_t b_base = (nr > 0 && B.off) ? B.off[row0] : 0;
+    // row0 <= U for every tile (ntiles = floor(U / RPT) + 1), so off[row0]
+    // exists; the trailing tile (nr == 0 when RPT divides U) then sees P = 0.
+    const int32_t a_base = A.off[row0];
+    const int32_t b_base = B.off ? B.off[row0] : 0;
     for (int r = tid; r <= nr; r += kBlock) {
         int32_t as = A.off[row0 + r] - a_base;
         int32_t bs = B.off ? B.off[row0 + r] - b_base : 0;

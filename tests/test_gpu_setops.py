@@ -170,3 +170,28 @@ def test_tile_fallback_long_rows(sx):
         for op in ("union", "intersection", "difference"):
             print("long rows", counts[:3], op, flush=True)
             assert_set_equal(to_host(_ops(sx)[op](da, db)), ox.setop(op, a, b))
+
+
+def test_row_count_multiple_of_tile_rows(sx):
+    """Row counts that the tile height divides (the launch then ends with an
+    empty tile): exact-capacity merges and binary operations stay exact.
+    Tile height = min(256, floor(1536 / mean intervals per row))."""
+    rng = np.random.default_rng(7)
+    for nrows, per_row in ((256, 1), (512, 1), (384, 4), (768, 2)):
+        off = np.arange(nrows + 1, dtype=np.int64) * per_row
+        beg, end = [], []
+        for _ in range(nrows):
+            x = int(rng.integers(0, 5))
+            for _ in range(per_row):
+                b = x + int(rng.integers(2, 9))
+                e = b + int(rng.integers(1, 9))
+                beg.append(b)
+                end.append(e)
+                x = e
+        a = ox.make(off.tolist(), beg, end)
+        b = ox.translate(a, 1, 0)
+        da, db = to_device(a), to_device(b)
+        print("rows", nrows, "per row", per_row, flush=True)
+        assert_set_equal(to_host(sx.dilate_interval_set(da, 1, 0)), ox.dilate(a, 1, 0))
+        for op in ("union", "intersection", "difference"):
+            assert_set_equal(to_host(_ops(sx)[op](da, db)), ox.setop(op, a, b))
