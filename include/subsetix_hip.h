@@ -155,6 +155,33 @@ int32_t stx_rows_in_range(const int32_t* rows, int64_t R, int32_t height, uint32
 int32_t stx_upwind(int32_t dtype, int32_t group, const int32_t* rows, const int32_t* off, int64_t R, const int32_t* beg,
                    const int32_t* end, const int32_t* co, const void* u, void* out, int32_t width, int32_t height,
                    double a, double b, double dt, double dx, double dy, hipStream_t stream);
+/* Segment form of the same update (interval_stencil.py:16-113).  A
+ * per-geometry table, built once and reused for every step on that geometry
+ * and domain width, cuts every interval into segments over which the
+ * intervals of the rows above and below do not change, also cut at columns 0
+ * and width (8 int32 per segment).  Build:
+ * stx_stencil_seg_count (per-interval counts, their scan into seg_off[N+1],
+ * total in *d_total), then stx_stencil_seg_build (records; row_inst / inst_w
+ * optional, for a stack of instances; plan = 2 int32 per tile of
+ * stx_stencil_tile_cells() cells, stx_stencil_plan_tiles(C) tiles).  co: N+1
+ * cell offsets; ord: row index of every interval.  Row ids must lie in
+ * [0, height).  stx_upwind_seg: inst_coef == NULL: one instance (width, a, b,
+ * dt, dx, dy as given; dtype 4 rounds them to float); otherwise the batched
+ * form of stx_upwind_batched with a table built with row_inst / inst_w. */
+int64_t stx_stencil_tile_cells(void);
+int64_t stx_stencil_plan_tiles(int64_t cells);
+int32_t stx_stencil_seg_count(const int32_t* rows, const int32_t* off, int64_t R, const int32_t* beg,
+                              const int32_t* end, const int32_t* co, const int32_t* ord, int64_t N, int32_t width,
+                              const int32_t* row_inst, const int32_t* inst_w, int32_t* counts, int32_t* seg_off,
+                              int32_t* d_total, uint32_t* status, void* scratch, int64_t scratch_bytes,
+                              uint32_t epoch, hipStream_t stream);
+int32_t stx_stencil_seg_build(const int32_t* rows, const int32_t* off, int64_t R, const int32_t* beg,
+                              const int32_t* end, const int32_t* co, const int32_t* ord, int64_t N, int32_t width,
+                              const int32_t* row_inst, const int32_t* inst_w, const int32_t* seg_off, int64_t nseg,
+                              int32_t* segs, int64_t C, int32_t* plan, int64_t ntiles, hipStream_t stream);
+int32_t stx_upwind_seg(int32_t dtype, const int32_t* plan, int64_t ntiles, int64_t C, const int32_t* segs,
+                       const void* u, void* out, double a, double b, double dt, double dx, double dy,
+                       const void* inst_coef, hipStream_t stream);
 /* FP64 mass sum(u)*scale with a fixed reduction order (no reference
  * counterpart; conservation check of the AMR step).  part: stx_mass_parts(n)
  * doubles. */

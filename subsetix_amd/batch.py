@@ -35,6 +35,7 @@ from . import _core, _lib
 from ._lib import _I32, _I64, _P, check, context, ptr
 from .expressions import IntervalSet, binary_op
 from .interval_field import IntervalField, cell_offsets, create_interval_field
+from .interval_stencil import build_stencil_plan
 from .levels import gradient_tag, prolong_into, restrict_into, upwind_batched
 from .morphology import dilate_interval_set
 from .multilevel import prolong_set, restrict_set
@@ -195,6 +196,10 @@ class BatchedAMR:
         self.row_inst = [self._row_inst(self.ext[lv], lv) for lv in range(L)]
         for t in self.ghosts[1:] + self.refines:
             cell_offsets(t)
+        # stencil tile plans (per-interval instance and width), built before
+        # any graph capture
+        self.plans = [build_stencil_plan(self.ext[lv], self.f[lv].interval_cell_offsets, 0, self.row_inst[lv],
+                                         self.inst_w[lv]) for lv in range(L)]
         self.graph = None
         self.cells = self.cells_per_step()
         torch.cuda.synchronize(self.dev)
@@ -226,7 +231,7 @@ class BatchedAMR:
             prolong_into(self.ghosts[lv], cur[lv], cur[lv - 1], R)
         for lv in range(L):
             upwind_batched(cur[lv], cur[lv].values, nxt[lv].values, self.row_inst[lv], self.inst_w[lv],
-                           self.inst_coef[lv])
+                           self.inst_coef[lv], plan=self.plans[lv])
         for lv in range(L - 1, 0, -1):
             restrict_into(self.refines[lv - 1], nxt[lv - 1], nxt[lv], R, accumulate=self.cfg.accumulate)
 

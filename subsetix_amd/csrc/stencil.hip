@@ -1,0 +1,422 @@
+// Segment-based first-order upwind stencil on an IntervalField
+// (interval_stencil.py:67-160, kernels.py:1108-1199).
+//
+// Cells are stored row by row, interval by interval.  A per-geometry table,
+// built once and reused for every step on that geometry, cuts every interval
+// into SEGMENTS: maximal runs of cells over which the interval of the row
+// below and the interval of the row above do not change.  Inside a segment
+// the vertical neighbours of cell c are cells c + dd and c + ud (constant
+// deltas, or none), and the horizontal neighbours are c - 1 / c + 1 except at
+// the segment ends, where a flag says whether the adjacent cell is the
+// neighbour (same interval, or a touching interval of the same row).  The
+// reference instead binary-searches the neighbour rows for every cell.
+//
+// Segments are also cut at the domain edges x = 0 and x = w (the row's width:
+// the call's width, or the instance width for a stack of instances), so the
+// zero-exterior range checks fold into per-segment flags.  Segment record (8
+// int32): c0, c_end, dd, ud, flags, instance, 0, 0.  flags bit 0: c0 has a
+// left neighbour (cell c0 - 1); bit 1: the segment lies in [0, w), so inner
+// cells have both horizontal neighbours; bit 2: c_end - 1 has a right
+// neighbour (cell c_end).  dd / ud = kNone when there is no vertical
+// neighbour (row absent, column not held, or segment outside [0, w)).
+//
+// The update kernel cuts the cells into tiles of kTileCells; a thread owns 8
+// consecutive cells (one segment lookup, vector loads and stores), the
+// block stages the tile's segment records in LDS.  Tiles are mapped to
+// blocks XCD-major (block b runs on XCD b % 8; XCD x gets the tiles
+// [x * per, (x + 1) * per)) so the rows above and below a tile are read
+// through the same L2.
+//
+// Precondition (checked by the Python layer unless disabled): row ids lie in
+// [0, height), so adjacency of row ids decides the vertical neighbours.
+#include <limits.h>
+
+#include "stx_common.h"
+#include "stx_host.h"
+
+using namespace stx;
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kCellsPerThread = 8;
+constexpr int kTileCells = kThreads * kCellsPerThread;
+constexpr int kSegCap = 512;
+constexpr int kXcds = 8;
+constexpr int32_t kNone = INT32_MIN;
+constexpr int kRec = 8;
+
+inline unsigned nblocks(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+// First interval of [j, jend) whose end is > x (jend when none).
+__device__ __forceinline__ int32_t first_ending_after(const int32_t* __restrict__ end, int32_t j, int32_t jend,
+                                                      int32_t x) {
+    int32_t lo = j, hi = jend;
+    while (lo < hi) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (end[mid] <= x) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+// Segments of interval i: count (WRITE = false, counts[i]) or records
+// (WRITE = true, at seg_off[i]).  Breakpoints: the boundaries of the
+// intervals of the rows directly below and above, and the domain edges 0 and
+// w (the row's width), so that the x-range checks of a segment's cells are
+// constant and can be folded into its flags.
+template <bool WRITE>
+__global__ void segments_kernel(const int32_t* __restrict__ rows, const int32_t* __restrict__ off, int64_t R,
+                                const int32_t* __restrict__ beg, const int32_t* __restrict__ end,
+                                const int32_t* __restrict__ co, const int32_t* __restrict__ ord, int64_t N,
+                                int32_t width, const int32_t* __restrict__ row_inst,
+                                const int32_t* __restrict__ inst_w, int32_t* __restrict__ counts,
+                                const int32_t* __restrict__ seg_off, int32_t* __restrict__ segs) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const int64_t k = ord[i];
+    const int32_t rid = rows[k];
+    const int32_t o0 = off[k], o1 = off[k + 1];
+    const int32_t bi = beg[i], ei = end[i], ci = co[i];
+    int32_t jd = 0, jde = 0, ju = 0, jue = 0;
+    if (k > 0 && rows[k - 1] == rid - 1) {
+        jde = o0;
+        jd = first_ending_after(end, off[k - 1], jde, bi);
+    }
+    if (k + 1 < R && rows[k + 1] == rid + 1) {
+        jue = off[k + 2];
+        ju = first_ending_after(end, o1, jue, bi);
+    }
+    const bool fl_first = i > o0 && end[i - 1] == bi;
+    const bool fr_last = i + 1 < o1 && beg[i + 1] == ei;
+    int32_t inst = 0, w = width;
+    if (row_inst) {
+        inst = row_inst[k];
+        w = inst_w[inst];
+    }
+    int32_t x = bi;
+    int32_t n = 0;
+    const int64_t base = WRITE ? (int64_t)seg_off[i] : 0;
+    while (x < ei) {
+        while (jd < jde && end[jd] <= x) ++jd;
+        while (ju < jue && end[ju] <= x) ++ju;
+        int32_t nx = ei;
+        if (jd < jde) {
+            const int32_t bd = x < beg[jd] ? beg[jd] : end[jd];
+            nx = bd < nx ? bd : nx;
+        }
+        if (ju < jue) {
+            const int32_t bu = x < beg[ju] ? beg[ju] : end[ju];
+            nx = bu < nx ? bu : nx;
+        }
+        if (x < 0 && 0 < nx) nx = 0;
+        if (x < w && w < nx) nx = w;
+        if (WRITE) {
+            int32_t* rec = segs + kRec * (base + n);
+            const int64_t cself = (int64_t)ci - bi;  // cell of column x is cself + x
+            const bool inside = x >= 0 && nx <= w;   // all of [x, nx) or none of it in [0, w)
+            const bool lf = (x == bi ? fl_first : true) && x - 1 >= 0 && x - 1 < w;
+            const bool rl = (nx == ei ? fr_last : true) && nx >= 0 && nx < w;
+            rec[0] = (int32_t)(cself + x);
+            rec[1] = (int32_t)(cself + nx);
+            rec[2] = (inside && jd < jde && beg[jd] <= x) ? (int32_t)(((int64_t)co[jd] - beg[jd]) - cself) : kNone;
+            rec[3] = (inside && ju < jue && beg[ju] <= x) ? (int32_t)(((int64_t)co[ju] - beg[ju]) - cself) : kNone;
+            rec[4] = (lf ? 1 : 0) | (inside ? 2 : 0) | (rl ? 4 : 0);
+            rec[5] = inst;
+            rec[6] = 0;
+            rec[7] = 0;
+        }
+        ++n;
+        x = nx;
+    }
+    if (!WRITE) counts[i] = n;
+}
+
+// plan[2 t], plan[2 t + 1] = segments [s0, s1) overlapping tile t.
+__global__ void seg_plan_kernel(const int32_t* __restrict__ segs, int64_t nseg, int64_t C, int64_t ntiles,
+                                int32_t* __restrict__ plan) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntiles) return;
+    const int64_t cf = t * kTileCells;
+    const int64_t cl = (cf + kTileCells < C ? cf + kTileCells : C) - 1;
+    auto locate = [&](int64_t c) {  // last segment with c0 <= c
+        int64_t lo = 0, hi = nseg;
+        while (hi - lo > 1) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (segs[kRec * mid] <= c) lo = mid; else hi = mid;
+        }
+        return lo;
+    };
+    plan[2 * t] = (int32_t)locate(cf);
+    plan[2 * t + 1] = (int32_t)(locate(cl) + 1);
+}
+
+struct Seg {
+    int32_t c0, cend, dd, ud, flags, inst;
+};
+
+struct GlobalSegs {
+    const int4* __restrict__ p;
+    __device__ __forceinline__ int32_t c0(int32_t s) const { return p[2 * s].x; }
+    __device__ __forceinline__ Seg get(int32_t s) const {
+        const int4 a = p[2 * s], b = p[2 * s + 1];
+        return Seg{a.x, a.y, a.z, a.w, b.x, b.y};
+    }
+};
+
+struct LdsSegs {
+    const int4* p;
+    int32_t base;
+    __device__ __forceinline__ int32_t c0(int32_t s) const { return p[2 * (s - base)].x; }
+    __device__ __forceinline__ Seg get(int32_t s) const {
+        const int4 a = p[2 * (s - base)], b = p[2 * (s - base) + 1];
+        return Seg{a.x, a.y, a.z, a.w, b.x, b.y};
+    }
+};
+
+template <typename T>
+struct Vec;
+template <>
+struct Vec<float> {
+    typedef float __attribute__((ext_vector_type(4))) type;
+    static constexpr int n = 4;
+};
+template <>
+struct Vec<double> {
+    typedef double __attribute__((ext_vector_type(2))) type;
+    static constexpr int n = 2;
+};
+
+template <typename T>
+__device__ __forceinline__ T upwind_cell(T center, T left, T right, T down, T up, T a, T b, T dt, T dx, T dy) {
+    const T nx = (a >= (T)0) ? center - left : right - center;
+    const T ny = (b >= (T)0) ? center - down : up - center;
+    const T dudx = nx / dx;
+    const T dudy = ny / dy;
+    const T t1 = a * dudx;
+    const T t2 = b * dudy;
+    return center - dt * (t1 + t2);
+}
+
+// Gathers go through a buffer resource over the whole field: a 32-bit byte
+// offset per load, and an index of -1 (or >= C) reads 0 (out-of-range buffer
+// loads return zero), which is exactly the zero exterior of the stencil.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t field_rsrc(const void* u, int64_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(u), (short)0, (int32_t)bytes, 0x00020000);
+}
+
+template <typename T>
+__device__ __forceinline__ T gather(__amdgpu_buffer_rsrc_t r, int32_t idx);
+template <>
+__device__ __forceinline__ float gather<float>(__amdgpu_buffer_rsrc_t r, int32_t idx) {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, idx * 4, 0, 0));
+}
+template <>
+__device__ __forceinline__ double gather<double>(__amdgpu_buffer_rsrc_t r, int32_t idx) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, idx * 8, 0, 0));
+}
+
+template <typename T>
+__device__ __forceinline__ typename Vec<T>::type load_dword_aligned(const T* p) {
+    typename Vec<T>::type v;
+    __builtin_memcpy(&v, __builtin_assume_aligned(p, 4), 16);
+    return v;
+}
+
+// One 16-byte group of G = 16 / sizeof(T) cells [c, c + G), c a multiple of
+// G.  Lanes of a wave own consecutive groups, so every vector load of the
+// wave covers one contiguous 1 KiB span.  Fast path (the group inside one
+// segment): one aligned load for the own values, one dword-aligned vector
+// load each for the cells below and above, two scalar loads for the cells
+// beside the group.  Otherwise a per-cell walk with scalar gathers.
+template <typename T, bool BATCH, class SG>
+__device__ __forceinline__ void upwind_group(const SG& sg, int32_t s0, int32_t s1, int32_t c, int32_t C,
+                                             __amdgpu_buffer_rsrc_t rs, const T* __restrict__ u, T* __restrict__ out,
+                                             T a, T b, T dt, T dx, T dy, const T* __restrict__ inst_coef) {
+    using VT = typename Vec<T>::type;
+    constexpr int G = Vec<T>::n;
+    int32_t lo = s0, hi = s1;
+    while (hi - lo > 1) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (sg.c0(mid) <= c) lo = mid; else hi = mid;
+    }
+    int32_t s = lo;
+    Seg g = sg.get(s);
+    if (c + G <= C && c + G <= g.cend) {
+        const VT cv = *(const VT*)(u + c);
+        VT dv = load_dword_aligned(u + (g.dd != kNone ? c + g.dd : c));
+        VT uv = load_dword_aligned(u + (g.ud != kNone ? c + g.ud : c));
+        const T lv = gather<T>(rs, c - 1);
+        const T rv = gather<T>(rs, c + G);
+        if (g.dd == kNone) dv = (VT)(T)0;
+        if (g.ud == kNone) uv = (VT)(T)0;
+        if (BATCH) {
+            const T* q = inst_coef + 5 * (int64_t)g.inst;
+            a = q[0];
+            b = q[1];
+            dt = q[2];
+            dx = q[3];
+            dy = q[4];
+        }
+        const bool inner = g.flags & 2;
+        const bool lfirst = c == g.c0 ? (g.flags & 1) : inner;
+        const bool rlast = c + G == g.cend ? (g.flags & 4) : inner;
+        VT ov;
+#pragma unroll
+        for (int v = 0; v < G; ++v) {
+            const bool lok = v == 0 ? lfirst : inner;
+            const bool rok = v == G - 1 ? rlast : inner;
+            const T left = lok ? (v == 0 ? lv : cv[v - 1]) : (T)0;
+            const T right = rok ? (v == G - 1 ? rv : cv[v + 1]) : (T)0;
+            ov[v] = upwind_cell<T>(cv[v], left, right, dv[v], uv[v], a, b, dt, dx, dy);
+        }
+        *(VT*)(out + c) = ov;
+        return;
+    }
+#pragma unroll
+    for (int v = 0; v < G; ++v) {
+        const int32_t cc = c + v;
+        if (cc < C) {
+            if (cc >= g.cend) {
+                ++s;
+                g = sg.get(s);
+            }
+            const bool lok = (cc == g.c0) ? (g.flags & 1) : (g.flags & 2);
+            const bool rok = (cc + 1 == g.cend) ? (g.flags & 4) : (g.flags & 2);
+            const T center = u[cc];
+            const T left = lok ? u[cc - 1] : (T)0;
+            const T right = rok ? u[cc + 1] : (T)0;
+            const T down = gather<T>(rs, g.dd != kNone ? cc + g.dd : -1);
+            const T up = gather<T>(rs, g.ud != kNone ? cc + g.ud : -1);
+            T ca = a, cb = b, cdt = dt, cdx = dx, cdy = dy;
+            if (BATCH) {
+                const T* q = inst_coef + 5 * (int64_t)g.inst;
+                ca = q[0];
+                cb = q[1];
+                cdt = q[2];
+                cdx = q[3];
+                cdy = q[4];
+            }
+            out[cc] = upwind_cell<T>(center, left, right, down, up, ca, cb, cdt, cdx, cdy);
+        }
+    }
+}
+
+template <typename T, bool BATCH, class SG>
+__device__ __forceinline__ void upwind_thread(const SG& sg, int32_t s0, int32_t s1, int32_t ct, int32_t C,
+                                              const T* __restrict__ u, T* __restrict__ out, T a, T b, T dt, T dx,
+                                              T dy, const T* __restrict__ inst_coef) {
+    constexpr int G = Vec<T>::n;
+    const __amdgpu_buffer_rsrc_t rs = field_rsrc(u, (int64_t)C * (int64_t)sizeof(T));
+#pragma unroll
+    for (int q = 0; q < kCellsPerThread / G; ++q) {
+        const int32_t c = ct + (q * kThreads + (int32_t)threadIdx.x) * G;
+        if (c < C) upwind_group<T, BATCH>(sg, s0, s1, c, C, rs, u, out, a, b, dt, dx, dy, inst_coef);
+    }
+}
+
+template <typename T, bool BATCH>
+__global__ __launch_bounds__(kThreads) void upwind_seg_kernel(const int32_t* __restrict__ plan, int64_t ntiles,
+                                                              int32_t C, const int32_t* __restrict__ segs,
+                                                              const T* __restrict__ u, T* __restrict__ out, T a,
+                                                              T b, T dt, T dx, T dy,
+                                                              const T* __restrict__ inst_coef) {
+    __shared__ int4 s_seg[2 * kSegCap];
+    const int64_t per = (int64_t)gridDim.x / kXcds;
+    const int64_t t = (int64_t)(blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+    if (t >= ntiles) return;
+    const int32_t s0 = plan[2 * t], s1 = plan[2 * t + 1];
+    const int32_t M = s1 - s0;
+    const int32_t ct = (int32_t)(t * kTileCells);
+    const int4* gs = (const int4*)segs;
+    if (M <= kSegCap) {
+        for (int q = threadIdx.x; q < 2 * M; q += kThreads) s_seg[q] = gs[2 * (int64_t)s0 + q];
+        __syncthreads();
+        upwind_thread<T, BATCH>(LdsSegs{s_seg, s0}, s0, s1, ct, C, u, out, a, b, dt, dx, dy, inst_coef);
+    } else {
+        upwind_thread<T, BATCH>(GlobalSegs{gs}, s0, s1, ct, C, u, out, a, b, dt, dx, dy, inst_coef);
+    }
+}
+
+template <typename T, bool BATCH>
+int32_t launch_upwind_seg(const int32_t* plan, int64_t ntiles, int64_t C, const int32_t* segs, const void* u,
+                          void* out, double a, double b, double dt, double dx, double dy, const void* inst_coef,
+                          hipStream_t stream) {
+    const int64_t grid = (ntiles + kXcds - 1) / kXcds * kXcds;
+    if (grid > 0x7fffffffLL) return stx_fail(STX_EINVAL, "upwind_seg: too many tiles");
+    if ((uintptr_t)u % 16 != 0 || (uintptr_t)out % 16 != 0)
+        return stx_fail(STX_EINVAL, "upwind_seg: values and output must be 16-byte aligned");
+    hipLaunchKernelGGL((upwind_seg_kernel<T, BATCH>), dim3((unsigned)grid), dim3(kThreads), 0, stream, plan, ntiles,
+                       (int32_t)C, segs, (const T*)u, (T*)out, (T)a, (T)b, (T)dt, (T)dx, (T)dy, (const T*)inst_coef);
+    return stx_launch_status("upwind_seg");
+}
+
+}  // namespace
+
+extern "C" int32_t stx_scan_i32(const int32_t* in, const int32_t* d_n, int64_t n_bound, int32_t* out, int32_t* d_total,
+                                uint32_t* status, void* scratch, int64_t scratch_bytes, uint32_t epoch,
+                                hipStream_t stream);
+
+extern "C" {
+
+int64_t stx_stencil_tile_cells(void) { return kTileCells; }
+
+int64_t stx_stencil_plan_tiles(int64_t cells) { return (cells + kTileCells - 1) / kTileCells; }
+
+int32_t stx_stencil_seg_count(const int32_t* rows, const int32_t* off, int64_t R, const int32_t* beg,
+                              const int32_t* end, const int32_t* co, const int32_t* ord, int64_t N, int32_t width,
+                              const int32_t* row_inst, const int32_t* inst_w, int32_t* counts, int32_t* seg_off,
+                              int32_t* d_total, uint32_t* status, void* scratch, int64_t scratch_bytes,
+                              uint32_t epoch, hipStream_t stream) {
+    if (N < 0 || R < 0) return stx_fail(STX_EINVAL, "stencil_seg_count: bad sizes");
+    if (N > 0 && R == 0) return stx_fail(STX_EINVAL, "stencil_seg_count: intervals without rows");
+    if ((row_inst == nullptr) != (inst_w == nullptr))
+        return stx_fail(STX_EINVAL, "stencil_seg_count: row_inst and inst_w go together");
+    if (row_inst == nullptr && width <= 0) return stx_fail(STX_EINVAL, "stencil_seg_count: width must be positive");
+    if (N > 0)
+        hipLaunchKernelGGL(segments_kernel<false>, dim3(nblocks(N)), dim3(256), 0, stream, rows, off, R, beg, end, co,
+                           ord, N, width, row_inst, inst_w, counts, nullptr, nullptr);
+    int32_t rc = stx_launch_status("stencil_seg_count");
+    if (rc) return rc;
+    return stx_scan_i32(counts, nullptr, N, seg_off, d_total, status, scratch, scratch_bytes, epoch, stream);
+}
+
+int32_t stx_stencil_seg_build(const int32_t* rows, const int32_t* off, int64_t R, const int32_t* beg,
+                              const int32_t* end, const int32_t* co, const int32_t* ord, int64_t N, int32_t width,
+                              const int32_t* row_inst, const int32_t* inst_w, const int32_t* seg_off, int64_t nseg,
+                              int32_t* segs, int64_t C, int32_t* plan, int64_t ntiles, hipStream_t stream) {
+    if (N < 0 || R < 0 || nseg < 0 || ntiles < 0) return stx_fail(STX_EINVAL, "stencil_seg_build: bad sizes");
+    if ((row_inst == nullptr) != (inst_w == nullptr))
+        return stx_fail(STX_EINVAL, "stencil_seg_build: row_inst and inst_w go together");
+    if (ntiles != (C + kTileCells - 1) / kTileCells) return stx_fail(STX_EINVAL, "stencil_seg_build: tile count");
+    if (N > 0)
+        hipLaunchKernelGGL(segments_kernel<true>, dim3(nblocks(N)), dim3(256), 0, stream, rows, off, R, beg, end, co,
+                           ord, N, width, row_inst, inst_w, nullptr, seg_off, segs);
+    if (ntiles > 0) {
+        if (nseg <= 0) return stx_fail(STX_EINVAL, "stencil_seg_build: cells without segments");
+        hipLaunchKernelGGL(seg_plan_kernel, dim3(nblocks(ntiles)), dim3(256), 0, stream, segs, nseg, C, ntiles, plan);
+    }
+    return stx_launch_status("stencil_seg_build");
+}
+
+int32_t stx_upwind_seg(int32_t dtype, const int32_t* plan, int64_t ntiles, int64_t C, const int32_t* segs,
+                       const void* u, void* out, double a, double b, double dt, double dx, double dy,
+                       const void* inst_coef, hipStream_t stream) {
+    if (ntiles <= 0 || C <= 0) return STX_OK;
+    if (C > 0x7fffffffLL - 2 * kTileCells) return stx_fail(STX_EINVAL, "upwind_seg: too many cells");
+    if (C * (dtype == 8 ? 8 : 4) > 0xffffff00LL)
+        return stx_fail(STX_EINVAL, "upwind_seg: field larger than 4 GiB (use the row-group kernel)");
+    if (ntiles != (C + kTileCells - 1) / kTileCells) return stx_fail(STX_EINVAL, "upwind_seg: plan size mismatch");
+    const bool batch = inst_coef != nullptr;
+    if (dtype == 4)
+        return batch ? launch_upwind_seg<float, true>(plan, ntiles, C, segs, u, out, a, b, dt, dx, dy, inst_coef, stream)
+                     : launch_upwind_seg<float, false>(plan, ntiles, C, segs, u, out, a, b, dt, dx, dy,
+                                                       inst_coef, stream);
+    if (dtype == 8)
+        return batch ? launch_upwind_seg<double, true>(plan, ntiles, C, segs, u, out, a, b, dt, dx, dy,
+                                                       inst_coef, stream)
+                     : launch_upwind_seg<double, false>(plan, ntiles, C, segs, u, out, a, b, dt, dx, dy,
+                                                        inst_coef, stream);
+    return stx_fail(STX_EINVAL, "upwind_seg: dtype must be 4 or 8");
+}
+
+}  // extern "C"

@@ -94,10 +94,19 @@ def gradient_tag(field: IntervalField, threshold: float) -> IntervalSet:
 
 
 def upwind_batched(field: IntervalField, values: torch.Tensor, out: torch.Tensor, row_inst: torch.Tensor,
-                   inst_w: torch.Tensor, inst_coef: torch.Tensor) -> torch.Tensor:
-    """One upwind step over a stack of instances (see stx_upwind_batched)."""
+                   inst_w: torch.Tensor, inst_coef: torch.Tensor, plan=None) -> torch.Tensor:
+    """One upwind step over a stack of instances (see stx_upwind_batched);
+    ``plan`` from interval_stencil.build_stencil_plan(set, co, row_inst,
+    inst_w) selects the segment kernel."""
+    from .interval_stencil import _segment_kernel_ok, stencil_schedule
+
     s = field.interval_set
     ctx = context(values.device)
+    if plan is not None and stencil_schedule() == "tiled" and _segment_kernel_ok(values, out):
+        p, ntiles, cells, segs = plan
+        check(_lib.lib().stx_upwind_seg(_dtype_code(values), ptr(p), ntiles, cells, ptr(segs), ptr(values), ptr(out),
+                                        0.0, 0.0, 0.0, 1.0, 1.0, ptr(inst_coef), ctx.stream), "upwind_batched")
+        return out
     check(_lib.lib().stx_upwind_batched(_dtype_code(values), _group(int(values.numel()), s.row_count), ptr(s.rows),
                                         ptr(s.row_offsets), s.row_count, ptr(s.begin), ptr(s.end),
                                         ptr(field.interval_cell_offsets), ptr(values), ptr(out), ptr(row_inst),

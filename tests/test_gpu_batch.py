@@ -92,6 +92,9 @@ def test_batched_upwind_matches_single_instance(torch_mod):
     from subsetix_amd.levels import upwind_batched
 
     upwind_batched(f, f.values, out, eng.row_inst[0], eng.inst_w[0], eng.inst_coef[0])
+    out_t = torch.empty_like(f.values)
+    upwind_batched(f, f.values, out_t, eng.row_inst[0], eng.inst_w[0], eng.inst_coef[0], plan=eng.plans[0])
+    np.testing.assert_array_equal(out_t.cpu().numpy(), out.cpu().numpy())
     off = 0
     for k, (inst, n) in enumerate(zip(insts, eng.sizes)):
         n = int(n)

@@ -292,3 +292,54 @@ def test_mass(sx):
             got = total_mass(t, 0.25)
             exp = ox.mass(u.astype(np.float32 if dt == torch.float32 else np.float64), 0.25)
             assert abs(got - exp) <= 1e-12 * max(1.0, abs(exp))
+
+
+def _many_small_intervals(rng, nrows, per_row, touching):
+    off, beg, end = [0], [], []
+    for _ in range(nrows):
+        x = int(rng.integers(-3, 3))
+        for _ in range(per_row):
+            b = x + (int(rng.integers(0, 3)) if touching else int(rng.integers(1, 3)))
+            e = b + int(rng.integers(1, 3))
+            beg.append(b)
+            end.append(e)
+            x = e
+        off.append(len(beg))
+    return ox.make(off, beg, end)
+
+
+@pytest.mark.parametrize("schedule", ["tiled", "rows"])
+def test_stencil_schedules_vs_oracle(sx, schedule):
+    """Both stencil schedules, bit-exact: dense rows spanning several cell
+    tiles; rows of ~500 tiny (touching) intervals whose tile neighbourhood
+    exceeds the LDS stage (global-memory path); rows with gaps."""
+    import torch
+
+    from subsetix_amd import interval_stencil as st
+    from subsetix_amd.interval_field import create_interval_field
+
+    rng = np.random.default_rng(11)
+    cases = [
+        (ox.make(list(range(65)), [0] * 64, [3000] * 64), 3000, 64),
+        (_many_small_intervals(rng, 12, 500, True), 1500, 12),
+        (_many_small_intervals(rng, 9, 700, False), 2600, 9),
+    ]
+    rows_gap = np.array([0, 1, 2, 5, 6, 9, 10, 11], np.int32)
+    cases.append((ox.make(list(range(0, 9)), [-2, 0, 5, 1, 0, 3, 0, 0], [40, 41, 60, 7, 50, 52, 30, 45],
+                          rows=rows_gap), 45, 12))
+    st.set_stencil_schedule(schedule)
+    try:
+        for s, w, h in cases:
+            ncell = int(ox.cell_offsets(s)[-1])
+            u = rng.random(ncell).astype(np.float32)
+            field = create_interval_field(to_device(s), 0.0, dtype=torch.float32)
+            field.values.copy_(torch.from_numpy(u))
+            for a, b in ((0.7, 0.4), (-0.3, -0.9)):
+                kw = dict(width=w, height=h, a=a, b=b, dt=0.01, dx=1.0 / w, dy=1.0 / h)
+                print(schedule, "cells", ncell, "a,b", a, b, flush=True)
+                got = st.step_upwind_interval(field, **kw).cpu().numpy()
+                np.testing.assert_array_equal(got, ox.upwind(s, u, **kw))
+                got64 = st.step_upwind_interval(field, compute_dtype=torch.float64, **kw).cpu().numpy()
+                np.testing.assert_array_equal(got64, ox.upwind(s, u.astype(np.float64), **kw))
+    finally:
+        st.set_stencil_schedule("tiled")
