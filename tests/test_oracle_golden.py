@@ -373,3 +373,19 @@ def test_upwind_f64_closed_form():
             left = row[k - 1] if k > 0 else 0.0
             exp.append(row[k] - 0.1 * (1.0 * (row[k] - left) + 1.0 * (row[k] - 0.0)))
     np.testing.assert_allclose(out, exp, rtol=0, atol=1e-15)
+
+
+def test_mass_closed_form():
+    """FP64 mass: sum(u) * area against an exactly rounded sum (math.fsum)
+    and against integer-valued data whose sum is exact."""
+    import math
+
+    rng = np.random.default_rng(9)
+    for n in (0, 1, 4095, 4096, 4097, 100003):
+        u = rng.random(n)
+        got = ox.mass(u, 0.25)
+        assert abs(got - math.fsum(u) * 0.25) <= 1e-12 * max(1.0, math.fsum(u))
+        u32 = u.astype(np.float32)
+        assert abs(ox.mass(u32, 0.5) - math.fsum(u32.astype(np.float64)) * 0.5) <= 1e-12 * max(1.0, n)
+    k = np.arange(1000, dtype=np.float64)
+    assert ox.mass(k, 2.0) == 2.0 * 999 * 1000 / 2
