@@ -15,6 +15,8 @@ Also reported beside the headline (each in its own object):
   batched_amr    512 three-level instances, 200 steps, regrid every 8, hipGraph
                  steps, vs the plain single-instance form on a sample
   plain_amr      reference-shaped two-level run, 128^2 Gaussian bump, FP64
+  serve          served stream (Poisson arrivals, continuous batching under an
+                 HBM budget) vs the unbatched plain loop: throughput, p50/p99
   roofline       dominant kernel (fused union) against 8 TB/s
   cpu_baseline   the C oracle (OpenMP over rows) on a bounded sample
 """
@@ -33,7 +35,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-SECTIONS = ("morphology", "large_stencil", "batched_amr", "plain_amr")
+SECTIONS = ("morphology", "large_stencil", "batched_amr", "plain_amr", "serve")
 
 
 def parse():
@@ -50,6 +52,8 @@ def parse():
     p.add_argument("--stencil-steps", type=int, default=500)
     p.add_argument("--amr-instances", type=int, default=512)
     p.add_argument("--amr-steps", type=int, default=200)
+    p.add_argument("--serve-rates", default="100,1000,10000", help="Poisson arrival rates (requests/s) for the serve section")
+    p.add_argument("--serve-requests", type=int, default=64)
     return p.parse_args()
 
 
@@ -112,6 +116,8 @@ def main():
             extra[s] = perf.batched_amr(args.amr_instances, args.amr_steps, dev)
         elif s == "plain_amr":
             extra[s] = perf.plain_two_level(dev)
+        elif s == "serve":
+            extra[s] = perf.serve([float(r) for r in args.serve_rates.split(",")], args.serve_requests, dev)
         torch.cuda.empty_cache()
         log(f"{s} ({time.time() - t0:.1f}s): {json.dumps(extra[s])}")
 

@@ -89,6 +89,36 @@ class CompositeTwoLevel:
         restrict_into(self.refine, self.coarse, self.fine_field, self.ratio, accumulate=self.accumulate)
         prolong_into(self.ghost, self.fine_field, self.coarse, self.ratio)
 
+    def prepare(self) -> None:
+        """Build every cached table a step uses (cell offsets of the transfer
+        targets, stencil segment tables), so that steps launch kernels only
+        and can be captured into a graph."""
+        from ..interval_field import cell_offsets
+        from ..interval_stencil import stencil_plan
+
+        for t in (self.ghost, self.refine):
+            if t.interval_count:
+                cell_offsets(t)
+        stencil_plan(self.coarse_set, self._c_co, self.width)
+        stencil_plan(self.ext, self._f_co, self.width * self.ratio)
+
+    def capture(self, stream: "torch.cuda.Stream") -> "torch.cuda.CUDAGraph":
+        """Graph of two steps from the current buffer (ends on it again)."""
+        from .._lib import context
+
+        self.prepare()
+        stream.wait_stream(torch.cuda.current_stream(stream.device))
+        with torch.cuda.stream(stream):
+            context(stream.device)  # per-stream context allocated outside the capture
+        g = torch.cuda.CUDAGraph()
+        start = self._cur
+        with torch.cuda.graph(g, stream=stream):
+            self.step()
+            self.step()
+        self._cur = start
+        torch.cuda.current_stream(stream.device).wait_stream(stream)
+        return g
+
     def step(self) -> None:
         a, b = self.velocity
         c, f = self.coarse, self.fine_field

@@ -343,3 +343,27 @@ def plain_two_level(dev, n: int = 128, steps: int = 100, regrid_every: int = 10)
             "refine_cells": int((refine.end - refine.begin).sum().item()) if refine.interval_count else 0,
             "fine_intervals": fine.interval_count, "fine_rows": fine.row_count,
             "desc": "AMR2Simulation (full-domain levels as the reference), FP64, value tag u>=0.1, moore grading 2"}
+
+
+def serve(rates, count: int, dev, seed: int = 5) -> Dict:
+    """Served stream at each Poisson rate: continuous batching server vs the
+    unbatched plain loop on the same request trace."""
+    from .serve import Server, make_trace, plain_loop
+
+    out = {"requests_per_rate": count, "mix": "70% advection (fine 256..4096 log-uniform, 10..200 steps), "
+                                             "30% set queries (rows 256..16384), HBM budget 200 GiB", "rates": {}}
+    # warm-up: kernels, graph capture path, allocator
+    Server(streams=4).run(make_trace(100.0, 4, seed=seed + 100, n_fine=(128, 256), steps=(4, 8), query_rows=(64, 128)))
+    for rate in rates:
+        trace = make_trace(float(rate), count, seed=seed)
+        t0 = time.perf_counter()
+        srv = Server(streams=4).run(trace).summary()
+        log(f"serve rate {rate}: server done in {time.perf_counter() - t0:.1f}s")
+        plain = plain_loop(trace).summary()
+        log(f"serve rate {rate}: plain loop done")
+        out["rates"][str(rate)] = {
+            "server": srv, "plain": plain,
+            "cell_updates_ratio": srv["cell_updates_per_s"] / max(plain["cell_updates_per_s"], 1e-9),
+            "throughput_ratio": srv["throughput_req_per_s"] / max(plain["throughput_req_per_s"], 1e-9),
+        }
+    return out

@@ -1,0 +1,400 @@
+"""Served stream: continuous batching of advection jobs and set-algebra
+queries on one GPU (no reference counterpart; the reference runs one
+simulation or one expression per call, simulation.py:232-354,
+expressions.py:446-528).
+
+Requests arrive on a Poisson clock.  The scheduler admits them in arrival
+order while their estimated device bytes fit the HBM budget (a request that
+does not fit waits; later requests do not overtake it).  The server loop then
+keeps every admitted request resident and, per iteration:
+
+  * set-algebra queries admitted since the last iteration are stacked row-wise
+    into one CSR pair (row ids shifted per query) and answered by ONE fused
+    launch per operation (union, intersection, difference); each query's
+    result is its slice of the stacked output;
+  * every resident advection job (a two-level composite instance) advances a
+    chunk of steps by replaying its own captured two-step graph; jobs are
+    spread over a few streams so small jobs overlap on the device.
+
+A job's result does not depend on what else is resident: queries are
+row-independent, and each job owns its fields and tables and runs
+deterministic kernels.  The plain loop (``plain_loop``) runs the same
+requests one at a time with eager launches and a host synchronisation per
+step, the reference-shaped form.
+"""
+
+from __future__ import annotations
+
+import math
+import time
+from collections import deque
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional
+
+import numpy as np
+import torch
+
+from . import _core, workloads
+from .expressions import IntervalSet
+
+HBM_BUDGET_BYTES = 200 * (1 << 30)
+
+
+# ---------------------------------------------------------------- requests
+@dataclass
+class Request:
+    rid: int
+    kind: str                 # "advect" or "query"
+    arrival: float            # seconds from the start of the run
+    payload: dict
+    est_bytes: int
+
+
+@dataclass
+class Result:
+    rid: int
+    kind: str
+    arrival: float
+    admitted: float
+    finished: float
+    output: dict = field(default_factory=dict)
+
+    @property
+    def latency(self) -> float:
+        return self.finished - self.arrival
+
+
+def _host_set(h) -> dict:
+    return {"rows": np.ascontiguousarray(h.rows), "off": np.ascontiguousarray(h.row_offsets),
+            "begin": np.ascontiguousarray(h.begin), "end": np.ascontiguousarray(h.end)}
+
+
+def advect_request(rid: int, arrival: float, *, n_fine: int, steps: int, seed: int,
+                   velocity=(1.0, 0.5), dtype=torch.float32) -> Request:
+    """A config-3-style instance: fine n_fine^2 domain covered ~30% by seeded
+    discs + annulus, coarse (n_fine/2)^2 full level, ratio 2, initial values
+    U[0, 1] (seeded), ``steps`` upwind + transfer steps."""
+    discs, scale, cov = workloads.stencil_discs_config(n=n_fine, seed=seed)
+    nc = n_fine // 2
+    fine_cells = 4 * int((discs.end.astype(np.int64) - discs.begin).sum())  # upper bound of prolong(restrict)
+    esize = 4 if dtype == torch.float32 else 8
+    est = 2 * esize * (nc * nc + int(1.2 * fine_cells)) + 64 * discs.n + (1 << 20)
+    return Request(rid, "advect", arrival, {"discs": _host_set(discs), "n_fine": n_fine, "steps": steps,
+                                            "seed": seed, "velocity": tuple(velocity), "dtype": dtype}, est)
+
+
+def query_request(rid: int, arrival: float, *, rows: int, seed: int, width: int = 65536) -> Request:
+    """Two seeded sets of the set-algebra distribution (Poisson(32) intervals
+    per row, geometric lengths / gaps 40 / 60) on ``rows`` rows."""
+    a = workloads.random_rows_set(rows, width, seed=seed)
+    b = workloads.random_rows_set(rows, width, seed=seed + 1)
+    est = 16 * (a.n + b.n) * 3 + 16 * rows * 4
+    return Request(rid, "query", arrival, {"a": _host_set(a), "b": _host_set(b), "rows": rows}, est)
+
+
+def make_trace(rate: float, count: int, *, seed: int = 5, advect_frac: float = 0.7, n_fine=(256, 4096),
+               steps=(10, 200), query_rows=(256, 16384)) -> List[Request]:
+    """Poisson arrivals at ``rate`` requests/s; advection jobs with fine
+    domains log-uniform in ``n_fine`` (even sizes) and steps uniform in
+    ``steps``; queries with rows uniform in ``query_rows``."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for rid in range(count):
+        t += float(rng.exponential(1.0 / rate))
+        if rng.random() < advect_frac:
+            nf = int(np.exp(rng.uniform(np.log(n_fine[0]), np.log(n_fine[1])))) // 2 * 2
+            out.append(advect_request(rid, t, n_fine=nf, steps=int(rng.integers(steps[0], steps[1] + 1)),
+                                      seed=int(rng.integers(1 << 30))))
+        else:
+            out.append(query_request(rid, t, rows=int(rng.integers(query_rows[0], query_rows[1] + 1)),
+                                     seed=int(rng.integers(1 << 30))))
+    return out
+
+
+# --------------------------------------------------------------- scheduler
+class Scheduler:
+    """FIFO admission under a device-byte budget."""
+
+    def __init__(self, budget_bytes: int = HBM_BUDGET_BYTES):
+        self.budget = int(budget_bytes)
+        self.used = 0
+        self.waiting: deque = deque()
+        self.peak = 0
+
+    def offer(self, req: Request) -> None:
+        if req.est_bytes > self.budget:
+            raise ValueError(f"request {req.rid} needs {req.est_bytes} bytes, more than the whole budget")
+        self.waiting.append(req)
+
+    def admit(self) -> List[Request]:
+        out = []
+        while self.waiting and self.used + self.waiting[0].est_bytes <= self.budget:
+            r = self.waiting.popleft()
+            self.used += r.est_bytes
+            out.append(r)
+        self.peak = max(self.peak, self.used)
+        return out
+
+    def release(self, req: Request) -> None:
+        self.used -= req.est_bytes
+
+
+# ------------------------------------------------------------ device state
+def _device_set(h: dict, dev, row_shift: int = 0) -> IntervalSet:
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev, non_blocking=False)
+    rows = h["rows"] + row_shift if row_shift else h["rows"]
+    return IntervalSet._trusted(t(h["begin"]), t(h["end"]), t(h["off"]), t(rows.astype(np.int32)), dense=None)
+
+
+class AdvectionJob:
+    def __init__(self, req: Request, dev, stream: Optional[torch.cuda.Stream], use_graph: bool):
+        from .amr2.composite import CompositeTwoLevel
+        from .multilevel import restrict_set
+
+        p = req.payload
+        self.req = req
+        self.remaining = int(p["steps"])
+        self.stream = stream
+        self.event: Optional[torch.cuda.Event] = None
+        discs = _device_set(p["discs"], dev)
+        nc = p["n_fine"] // 2
+        self.comp = CompositeTwoLevel(nc, nc, restrict_set(discs, 2), dtype=p["dtype"], velocity=p["velocity"])
+        # initial values U[0, 1) from a seeded device generator (same values
+        # whichever stream or server runs the job)
+        gen = torch.Generator(device=dev)
+        gen.manual_seed(int(p["seed"]))
+        cv = torch.rand(nc * nc, generator=gen, device=dev, dtype=torch.float32)
+        fv = torch.rand(self.comp.fine_field.values.numel(), generator=gen, device=dev, dtype=torch.float32)
+        self.comp.set_values(cv, fv)
+        self.graph = self.comp.capture(stream) if (use_graph and stream is not None and self.remaining >= 2) else None
+        self.graph_parity = self.comp._cur
+        self.cells_per_step = self.comp.stats().cell_updates_per_step
+
+    def advance(self, chunk: int) -> int:
+        """Launch up to ``chunk`` steps (no host synchronisation)."""
+        n = min(chunk, self.remaining)
+        done = 0
+        if self.graph is not None and self.comp._cur == self.graph_parity:
+            while n - done >= 2:
+                self.graph.replay()
+                done += 2
+        while done < n:
+            self.comp.step()
+            done += 1
+        self.remaining -= n
+        return n
+
+    def output(self) -> dict:
+        return {"mass": self.comp.mass(), "coarse": self.comp.coarse.values, "fine": self.comp.fine_field.values}
+
+
+def _stack_queries(reqs: List[Request], dev):
+    """Row-stack the A and B sets of several queries (16 empty rows between
+    queries); returns device sets and each query's row range."""
+    gap = 16
+    shift = 0
+    parts_a, parts_b, ranges = [], [], []
+    for r in reqs:
+        a, b = r.payload["a"], r.payload["b"]
+        parts_a.append((a, shift))
+        parts_b.append((b, shift))
+        ranges.append((shift, r.payload["rows"]))
+        shift += r.payload["rows"] + gap
+
+    def cat(parts):
+        rows = np.concatenate([h["rows"] + s for h, s in parts]).astype(np.int32)
+        beg = np.concatenate([h["begin"] for h, _ in parts])
+        end = np.concatenate([h["end"] for h, _ in parts])
+        offs, base = [np.zeros(1, np.int64)], 0
+        for h, _ in parts:
+            offs.append(h["off"][1:].astype(np.int64) + base)
+            base += int(h["off"][-1])
+        off = np.concatenate(offs).astype(np.int32)
+        return _device_set({"rows": rows, "off": off, "begin": beg, "end": end}, dev)
+
+    return cat(parts_a), cat(parts_b), ranges
+
+
+def run_queries(reqs: List[Request], dev) -> List[dict]:
+    """All three operations for a batch of queries: one fused launch per
+    operation over the row-stacked pairs (rows of the pairs are identical per
+    query, so no row alignment is needed).  Returns per-query host results."""
+    if not reqs:
+        return []
+    A, B, ranges = _stack_queries(reqs, dev)
+    R = A.row_count
+    outs = {}
+    for op, name in ((0, "union"), (1, "intersection"), (2, "difference")):
+        out_off, ob, oe, ctx = _core.setop_raw(op, (A.begin, A.end, A.row_offsets), (B.begin, B.end, B.row_offsets),
+                                               nrows=R)
+        (total,) = _core.read_counts(ctx, ctx.words[1])
+        outs[name] = (out_off.cpu().numpy(), ob[:total].cpu().numpy(), oe[:total].cpu().numpy())
+    res = []
+    k = 0
+    for (r0, nr) in ranges:
+        one = {}
+        for name, (off, ob, oe) in outs.items():
+            o = off[k:k + nr + 1]
+            one[name] = (o - o[0], ob[o[0]:o[-1]], oe[o[0]:o[-1]])
+        res.append(one)
+        k += nr + 16
+    return res
+
+
+# ------------------------------------------------------------------ server
+@dataclass
+class ServeStats:
+    results: List[Result]
+    wall: float
+    cell_updates: int
+    query_intervals: int
+    iterations: int
+    peak_bytes: int
+    setup_s: float = 0.0
+
+    def summary(self) -> dict:
+        lat = {k: sorted(r.latency for r in self.results if r.kind == k) for k in ("advect", "query")}
+        allat = sorted(r.latency for r in self.results)
+
+        def pct(v, q):
+            return float(np.percentile(v, q)) if v else None
+
+        return {
+            "requests": len(self.results), "wall_s": self.wall,
+            "throughput_req_per_s": len(self.results) / self.wall,
+            "cell_updates_per_s": self.cell_updates / self.wall,
+            "query_interval_ops_per_s": self.query_intervals / self.wall,
+            "latency_p50_s": pct(allat, 50), "latency_p99_s": pct(allat, 99),
+            "advect_p50_s": pct(lat["advect"], 50), "advect_p99_s": pct(lat["advect"], 99),
+            "query_p50_s": pct(lat["query"], 50), "query_p99_s": pct(lat["query"], 99),
+            "iterations": self.iterations, "peak_admitted_bytes": self.peak_bytes, "job_setup_s": self.setup_s,
+        }
+
+
+class Server:
+    """Continuous batching server (see the module docstring)."""
+
+    def __init__(self, *, budget_bytes: int = HBM_BUDGET_BYTES, streams: int = 4, chunk: int = 8,
+                 use_graph: bool = True, device=None, keep_outputs: bool = False):
+        self.dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+        self.sched = Scheduler(budget_bytes)
+        self.streams = [torch.cuda.Stream(self.dev) for _ in range(max(1, streams))]
+        self.chunk = int(chunk) + (int(chunk) % 2)  # even: graphs replay pairs of steps
+        self.use_graph = use_graph
+        self.keep_outputs = keep_outputs
+
+    def run(self, trace: List[Request]) -> ServeStats:
+        pending = deque(sorted(trace, key=lambda r: r.arrival))
+        active: List[AdvectionJob] = []
+        admitted_at: Dict[int, float] = {}
+        results: List[Result] = []
+        cell_updates = 0
+        qints = 0
+        iters = 0
+        rr = 0
+        setup = 0.0
+        main = torch.cuda.current_stream(self.dev)
+        torch.cuda.synchronize(self.dev)
+        t0 = time.perf_counter()
+        now = lambda: time.perf_counter() - t0
+
+        def launch(job: AdvectionJob) -> int:
+            job.stream.wait_stream(main)
+            with torch.cuda.stream(job.stream):
+                n = job.advance(self.chunk)
+                job.event = torch.cuda.Event()
+                job.event.record(job.stream)
+            return n
+
+        while pending or active or self.sched.waiting:
+            t = now()
+            while pending and pending[0].arrival <= t:
+                self.sched.offer(pending.popleft())
+            queries = []
+            progressed = False
+            for r in self.sched.admit():
+                admitted_at[r.rid] = now()
+                if r.kind == "query":
+                    queries.append(r)
+                else:
+                    s = self.streams[rr % len(self.streams)]
+                    rr += 1
+                    ts = time.perf_counter()
+                    job = AdvectionJob(r, self.dev, s, self.use_graph)
+                    setup += time.perf_counter() - ts
+                    cell_updates += launch(job) * job.cells_per_step  # first chunk right away
+                    active.append(job)
+                    progressed = True
+            if queries:
+                iters += 1
+                outs = run_queries(queries, self.dev)
+                t_q = now()
+                for r, o in zip(queries, outs):
+                    qints += 3 * (int(r.payload["a"]["off"][-1]) + int(r.payload["b"]["off"][-1]))
+                    results.append(Result(r.rid, "query", r.arrival, admitted_at[r.rid], t_q,
+                                          o if self.keep_outputs else {}))
+                    self.sched.release(r)
+                progressed = True
+            # jobs whose last chunk finished: next chunk, or completion
+            still = []
+            for job in active:
+                if job.event is not None and not job.event.query():
+                    still.append(job)
+                    continue
+                if job.remaining > 0:
+                    iters += 1
+                    cell_updates += launch(job) * job.cells_per_step
+                    still.append(job)
+                else:
+                    with torch.cuda.stream(job.stream):
+                        out = job.output()
+                        out = ({"mass": out["mass"], "coarse": out["coarse"].cpu().numpy(),
+                                "fine": out["fine"].cpu().numpy()} if self.keep_outputs else {"mass": out["mass"]})
+                    results.append(Result(job.req.rid, "advect", job.req.arrival, admitted_at[job.req.rid],
+                                          now(), out))
+                    self.sched.release(job.req)
+                progressed = True
+            active = still
+            if not progressed:
+                nxt = pending[0].arrival - now() if pending else 0.0
+                time.sleep(min(max(nxt, 0.0), 2e-4) if (active or nxt > 0) else 0.0)
+        torch.cuda.synchronize(self.dev)
+        wall = now()
+        st = ServeStats(sorted(results, key=lambda r: r.rid), wall, cell_updates, qints, iters, self.sched.peak)
+        st.setup_s = setup
+        return st
+
+
+def plain_loop(trace: List[Request], device=None, keep_outputs: bool = False) -> ServeStats:
+    """Unbatched plain form: requests one at a time in arrival order, eager
+    launches, a host synchronisation per step (advection) and per query."""
+    dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+    results = []
+    cell_updates = 0
+    qints = 0
+    setup = 0.0
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    now = lambda: time.perf_counter() - t0
+    for r in sorted(trace, key=lambda r: r.arrival):
+        while now() < r.arrival:
+            time.sleep(min(r.arrival - now(), 0.01))
+        ta = now()
+        if r.kind == "query":
+            o = run_queries([r], dev)[0]
+            qints += 3 * (int(r.payload["a"]["off"][-1]) + int(r.payload["b"]["off"][-1]))
+            results.append(Result(r.rid, "query", r.arrival, ta, now(), o if keep_outputs else {}))
+        else:
+            ts = time.perf_counter()
+            job = AdvectionJob(r, dev, None, False)
+            setup += time.perf_counter() - ts
+            while job.remaining:
+                job.advance(1)
+                torch.cuda.synchronize(dev)
+                cell_updates += job.cells_per_step
+            out = job.output()
+            out = ({"mass": out["mass"], "coarse": out["coarse"].cpu().numpy(), "fine": out["fine"].cpu().numpy()}
+                   if keep_outputs else {"mass": out["mass"]})
+            results.append(Result(r.rid, "advect", r.arrival, ta, now(), out))
+    return ServeStats(results, now(), cell_updates, qints, len(trace), 0, setup)
