@@ -244,24 +244,41 @@ __device__ __forceinline__ SegMax block_seg_max_exclusive(SegMax x, SegMax* lds 
     return seg_combine(carry, ex);
 }
 
+// s_k[idx] when ok, else (0, INT_MIN); the LDS read is unconditional (index
+// 0 when not ok) so the select is on values, never on addresses.
+__device__ __forceinline__ int2 lds_or_none(const int2* s, int32_t idx, bool ok) {
+    const int2 v = s[ok ? idx : 0];
+    return ok ? v : make_int2(0, INT_MIN);
+}
+
 template <int OP>
 __device__ __forceinline__ bool key_a_first(int32_t akb, int32_t bkb, int32_t bke) {
     if (OP == 2) return akb < bke;
     return akb <= bkb;
 }
 
-template <int OP>
+// MODE: kFused = count, look-back, write in one launch.  Two-pass form:
+// kCount publishes the tile's output count as its look-back state word
+// (aggregate) and stops; prefix_states_kernel turns all aggregates into
+// inclusive prefixes; kWrite reads tile - 1's prefix (one load, never a
+// wait) and writes.  It trades a second read of the inputs for tiles that
+// never wait on each other; every scratch word stays epoch-tagged.
+constexpr int kFused = 0, kCount = 1, kWrite = 2;
+
+template <int OP, int MODE, bool XF>
 __global__ __launch_bounds__(kBlock) void setop_tile_kernel(Operand A, Operand B, int64_t U, int32_t RPT, uint32_t ntiles,
                                                             int32_t* __restrict__ out_off, int32_t* __restrict__ ob,
                                                             int32_t* __restrict__ oe, int64_t capacity,
                                                             int32_t* __restrict__ d_total, uint32_t* status,
                                                             void* scratch, uint32_t epoch, int64_t spin_limit) {
-    __shared__ int32_t s_kb[kPCAP];
-    __shared__ int32_t s_ke[kPCAP];
-    __shared__ int32_t s_ex[kPCAP + 1];
+    // LDS kept under 20 KiB so 8 blocks (32 waves) fit on a CU
+    __shared__ int2 s_k[kPCAP];  // (begin, end) of the tile's A then B intervals
     __shared__ int32_t s_aS[kMaxRPT + 1];
     __shared__ int32_t s_bS[kMaxRPT + 1];
-    __shared__ int32_t s_M[kMaxRPT + 1];
+    // per thread: emissions before its first position, and its emission mask
+    // (fallback path: per-row counts, then their exclusive scan)
+    __shared__ int32_t s_tbase[kBlock + 1];
+    __shared__ uint8_t s_tmask[kBlock];
     __shared__ int32_t s_scan[17];
     __shared__ SegMax s_seg[kWaves];
     __shared__ uint32_t s_excl;
@@ -281,7 +298,6 @@ __global__ __launch_bounds__(kBlock) void setop_tile_kernel(Operand A, Operand B
         int32_t bs = B.off ? B.off[row0 + r] - b_base : 0;
         s_aS[r] = as;
         s_bS[r] = bs;
-        s_M[r] = as + bs;
     }
     __syncthreads();
     const int32_t nA = s_aS[nr], nB = s_bS[nr];
@@ -296,13 +312,12 @@ __global__ __launch_bounds__(kBlock) void setop_tile_kernel(Operand A, Operand B
     int32_t my_base = 0;
 
     if (fits) {
-        for (int k = tid; k < nA; k += kBlock) {
-            s_kb[k] = keyb(A, a_base + k);
-            s_ke[k] = keye(A, a_base + k);
-        }
-        for (int k = tid; k < nB; k += kBlock) {
-            s_kb[nA + k] = keyb(B, b_base + k);
-            s_ke[nA + k] = keye(B, b_base + k);
+        if (XF) {
+            for (int k = tid; k < nA; k += kBlock) s_k[k] = make_int2(keyb(A, a_base + k), keye(A, a_base + k));
+            for (int k = tid; k < nB; k += kBlock) s_k[nA + k] = make_int2(keyb(B, b_base + k), keye(B, b_base + k));
+        } else {
+            for (int k = tid; k < nA; k += kBlock) s_k[k] = make_int2(A.beg[a_base + k], A.end[a_base + k]);
+            for (int k = tid; k < nB; k += kBlock) s_k[nA + k] = make_int2(B.beg[b_base + k], B.end[b_base + k]);
         }
         __syncthreads();
         const int32_t p0 = tid * kVT;
@@ -311,22 +326,29 @@ __global__ __launch_bounds__(kBlock) void setop_tile_kernel(Operand A, Operand B
             int lo = 0, hi = nr;
             while (hi - lo > 1) {
                 int mid = (lo + hi) >> 1;
-                if (s_M[mid] <= p0) lo = mid; else hi = mid;
+                if (s_aS[mid] + s_bS[mid] <= p0) lo = mid; else hi = mid;
             }
             int r = lo;
             int32_t aS = s_aS[r], na = s_aS[r + 1] - aS;
             int32_t bS = nA + s_bS[r], nb = s_bS[r + 1] - s_bS[r];
-            int32_t rowend = s_M[r + 1];
+            int32_t rs = s_aS[r] + s_bS[r];
+            int32_t rowend = s_aS[r + 1] + s_bS[r + 1];
             // co-rank inside the row
-            int32_t l = p0 - s_M[r];
+            int32_t l = p0 - rs;
             int32_t clo = l - nb > 0 ? l - nb : 0, chi = l < na ? l : na;
             while (clo < chi) {
                 int32_t mid = (clo + chi) >> 1;
-                int32_t bj = bS + (l - 1 - mid);
-                if (key_a_first<OP>(s_kb[aS + mid], s_kb[bj], s_ke[bj])) clo = mid + 1;
+                const int2 kb = s_k[bS + (l - 1 - mid)];
+                if (key_a_first<OP>(s_k[aS + mid].x, kb.x, kb.y)) clo = mid + 1;
                 else chi = mid;
             }
             int32_t i = clo, j = l - clo;
+            // the walk keeps the current and previous element of each operand
+            // in registers: one LDS read per consumed element
+            int2 curA = lds_or_none(s_k, aS + i, i < na);
+            int2 curB = lds_or_none(s_k, bS + j, j < nb);
+            int2 prevA = lds_or_none(s_k, aS + i - 1, i >= 1);
+            int2 prevB = lds_or_none(s_k, bS + j - 1, j >= 1);
 #pragma unroll
             for (int v = 0; v < kVT; ++v) {
                 const int32_t p = p0 + v;
@@ -339,54 +361,57 @@ __global__ __launch_bounds__(kBlock) void setop_tile_kernel(Operand A, Operand B
                         na = s_aS[r + 1] - aS;
                         bS = nA + s_bS[r];
                         nb = s_bS[r + 1] - s_bS[r];
-                        rowend = s_M[r + 1];
+                        rs = rowend;
+                        rowend = s_aS[r + 1] + s_bS[r + 1];
                         i = 0;
                         j = 0;
+                        curA = lds_or_none(s_k, aS, na > 0);
+                        curB = lds_or_none(s_k, bS, nb > 0);
                     }
-                    if (p == s_M[r]) rowstart_mask |= 1u << v;
+                    if (p == rs) rowstart_mask |= 1u << v;
                     if (p == rowend - 1) rowend_mask |= 1u << v;
-                    const bool isA = (j >= nb) || (i < na && key_a_first<OP>(s_kb[aS + i], s_kb[bS + j], s_ke[bS + j]));
+                    const bool isA = (j >= nb) || (i < na && key_a_first<OP>(curA.x, curB.x, curB.y));
                     if (OP == 0) {
-                        int32_t eb, ee;
-                        bool clip;
-                        if (isA) { eb = s_kb[aS + i]; ee = s_ke[aS + i]; clip = A.xf.mode == XF_CLIP; }
-                        else { eb = s_kb[bS + j]; ee = s_ke[bS + j]; clip = B.xf.mode == XF_CLIP; }
-                        if (!(clip && eb >= ee)) alive_mask |= 1u << v;
-                        rb[v] = eb;
-                        re[v] = ee;
+                        const int2 e = isA ? curA : curB;
+                        const bool clip = XF && (isA ? A.xf.mode == XF_CLIP : B.xf.mode == XF_CLIP);
+                        if (!(clip && e.x >= e.y)) alive_mask |= 1u << v;
+                        rb[v] = e.x;
+                        re[v] = e.y;
                     } else if (OP == 1) {
                         if (isA) {
-                            const int32_t ab = s_kb[aS + i], ae = s_ke[aS + i];
+                            const int32_t ab = curA.x, ae = curA.y;
                             int32_t ce = INT_MIN;
-                            if (j >= 1 && s_ke[bS + j - 1] > ab) ce = s_ke[bS + j - 1];
-                            else if (j < nb && s_kb[bS + j] == ab) ce = s_ke[bS + j];
+                            if (j >= 1 && prevB.y > ab) ce = prevB.y;
+                            else if (j < nb && curB.x == ab) ce = curB.y;
                             if (ce != INT_MIN) { emit_mask |= 1u << v; rb[v] = ab; re[v] = ae < ce ? ae : ce; }
                         } else {
-                            const int32_t bb = s_kb[bS + j], be = s_ke[bS + j];
-                            if (i >= 1) {
-                                const int32_t pab = s_kb[aS + i - 1], pae = s_ke[aS + i - 1];
-                                if (pab < bb && bb < pae) { emit_mask |= 1u << v; rb[v] = bb; re[v] = be < pae ? be : pae; }
+                            const int32_t bb = curB.x, be = curB.y;
+                            if (i >= 1 && prevA.x < bb && bb < prevA.y) {
+                                emit_mask |= 1u << v;
+                                rb[v] = bb;
+                                re[v] = be < prevA.y ? be : prevA.y;
                             }
                         }
                     } else {
                         if (isA) {
-                            const int32_t ab = s_kb[aS + i], ae = s_ke[aS + i];
-                            const bool covered = j < nb && s_kb[bS + j] <= ab;
+                            const int32_t ab = curA.x, ae = curA.y;
+                            const bool covered = j < nb && curB.x <= ab;
                             if (!covered) {
                                 int32_t pe = ae;
-                                if (j < nb) { const int32_t nbg = s_kb[bS + j]; pe = pe < nbg ? pe : nbg; }
+                                if (j < nb) pe = pe < curB.x ? pe : curB.x;
                                 emit_mask |= 1u << v;
                                 rb[v] = ab;
                                 re[v] = pe;
                             }
                         } else {
-                            const int32_t be = s_ke[bS + j];
+                            const int32_t be = curB.y;
                             if (i >= 1) {
-                                const int32_t pae = s_ke[aS + i - 1];
-                                const bool open = (j + 1 >= nb) || s_kb[bS + j + 1] > be;
+                                const int32_t pae = prevA.y;
+                                const int2 nxt = lds_or_none(s_k, bS + j + 1, j + 1 < nb);
+                                const bool open = (j + 1 >= nb) || nxt.x > be;
                                 if (be < pae && open) {
                                     int32_t pe = pae;
-                                    if (j + 1 < nb) { const int32_t nbg = s_kb[bS + j + 1]; pe = pe < nbg ? pe : nbg; }
+                                    if (j + 1 < nb) pe = pe < nxt.x ? pe : nxt.x;
                                     emit_mask |= 1u << v;
                                     rb[v] = be;
                                     re[v] = pe;
@@ -394,7 +419,15 @@ __global__ __launch_bounds__(kBlock) void setop_tile_kernel(Operand A, Operand B
                             }
                         }
                     }
-                    if (isA) ++i; else ++j;
+                    if (isA) {
+                        prevA = curA;
+                        ++i;
+                        curA = lds_or_none(s_k, aS + i, i < na);
+                    } else {
+                        prevB = curB;
+                        ++j;
+                        curB = lds_or_none(s_k, bS + j, j < nb);
+                    }
                 }
             }
         }
@@ -418,45 +451,54 @@ __global__ __launch_bounds__(kBlock) void setop_tile_kernel(Operand A, Operand B
         }
         const int32_t my_count = __popc(emit_mask);
         my_base = block_exclusive_scan<int32_t>(my_count, s_scan, &tile_total);
-        {
-            int32_t run = my_base;
-            const int32_t p0 = tid * kVT;
-#pragma unroll
-            for (int v = 0; v < kVT; ++v) {
-                if (p0 + v < P) s_ex[p0 + v] = run;
-                if (emit_mask & (1u << v)) ++run;
-            }
-            if (tid == 0) s_ex[P] = tile_total;
-        }
+        s_tbase[tid] = my_base;
+        s_tmask[tid] = (uint8_t)emit_mask;
     } else {
-        // fallback: one wavefront per row, counts into s_ex[r]
+        // fallback: one wavefront per row, counts into s_tbase[r]
         const int wave = wave_id();
         for (int r = wave; r < nr; r += kWaves) {
             int32_t a0, na, b0, nb;
             row_extent(A, row0 + r, a0, na);
             row_extent(B, row0 + r, b0, nb);
             int32_t c = setop_row<64, OP, false>(A, a0, na, B, b0, nb, nullptr, nullptr, 0);
-            if (lane_id() == 0) s_ex[r] = c;
+            if (lane_id() == 0) s_tbase[r] = c;
         }
         __syncthreads();
-        int32_t v = tid < nr ? s_ex[tid] : 0;
+        int32_t v = tid < nr ? s_tbase[tid] : 0;
         int32_t ex = 0;
         // nr <= kMaxRPT == kBlock
         ex = block_exclusive_scan<int32_t>(v, s_scan, &tile_total);
         __syncthreads();
-        if (tid < nr) s_ex[tid] = ex;
-        if (tid == 0) s_ex[nr] = tile_total;
+        if (tid < nr) s_tbase[tid] = ex;
+        if (tid == 0) s_tbase[nr] = tile_total;
     }
-    if (wave_id() == 0) {
+    if (MODE == kCount) {
+        if (tid == 0) state_store(&lb.state[tile], epoch, 1u, (uint32_t)tile_total);
+        return;
+    }
+    if (MODE == kFused && wave_id() == 0) {
         uint32_t e = lookback_exclusive(lb, tile, (uint32_t)tile_total, epoch, status, spin_limit);
         if (lane_id() == 0) s_excl = e;
     }
     __syncthreads();
-    const int64_t excl = s_excl;
+    const int64_t excl =
+        MODE == kFused ? (int64_t)s_excl : (tile == 0 ? 0 : (int64_t)(uint32_t)state_load(&lb.state[tile - 1]));
     // row offsets
     for (int r = tid; r <= nr; r += kBlock) {
         if (r == nr && row0 + nr != U) break;
-        int32_t local = fits ? s_ex[s_M[r]] : s_ex[r];
+        int32_t local;
+        if (fits) {
+            // emissions before the row's first position q
+            const int32_t q = s_aS[r] + s_bS[r];
+            if (q >= P) {
+                local = tile_total;
+            } else {
+                const int t = q / kVT;
+                local = s_tbase[t] + __popc((uint32_t)s_tmask[t] & ((1u << (q % kVT)) - 1u));
+            }
+        } else {
+            local = s_tbase[r];
+        }
         out_off[row0 + r] = (int32_t)(excl + local);
         if (r == nr && d_total) *d_total = (int32_t)(excl + local);
     }
@@ -503,7 +545,47 @@ __global__ __launch_bounds__(kBlock) void setop_tile_kernel(Operand A, Operand B
             int32_t a0, na, b0, nb;
             row_extent(A, row0 + r, a0, na);
             row_extent(B, row0 + r, b0, nb);
-            setop_row<64, OP, true>(A, a0, na, B, b0, nb, ob, oe, excl + s_ex[r]);
+            setop_row<64, OP, true>(A, a0, na, B, b0, nb, ob, oe, excl + s_tbase[r]);
+        }
+    }
+}
+
+// One block: aggregates (flag 1) of tiles [0, ntiles) -> inclusive prefixes
+// (flag 2), same epoch.
+constexpr int kPrefixThreads = 1024;
+
+__global__ __launch_bounds__(kPrefixThreads) void prefix_states_kernel(void* scratch, uint32_t ntiles, uint32_t epoch) {
+    __shared__ uint32_t s_scan[kPrefixThreads / 64 + 1];
+    LookbackScratch lb = lookback_scratch(scratch, ntiles);
+    const uint32_t per = (ntiles + kPrefixThreads - 1) / kPrefixThreads;
+    const uint32_t t0 = threadIdx.x * per;
+    uint32_t sum = 0;
+    for (uint32_t k = 0; k < per; ++k)
+        if (t0 + k < ntiles) sum += (uint32_t)state_load(&lb.state[t0 + k]);
+    // block exclusive scan of the per-thread sums
+    const int lane = lane_id(), wave = wave_id();
+    uint32_t incl = sum;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t o = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += o;
+    }
+    if (lane == 63) s_scan[wave] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (int w = 0; w < kPrefixThreads / 64; ++w) {
+            const uint32_t v = s_scan[w];
+            s_scan[w] = acc;
+            acc += v;
+        }
+    }
+    __syncthreads();
+    uint32_t run = s_scan[wave] + incl - sum;
+    for (uint32_t k = 0; k < per; ++k) {
+        if (t0 + k < ntiles) {
+            run += (uint32_t)state_load(&lb.state[t0 + k]);
+            state_store(&lb.state[t0 + k], epoch, 2u, run);
         }
     }
 }
@@ -629,6 +711,37 @@ static int32_t setop_tile_rows(int64_t elems, int64_t rows) {
     return (int32_t)r;
 }
 
+constexpr uint32_t kTwoPassTiles = 0xffffffffu;  // measured slower than one pass at every size tried
+
+template <int OP, int MODE>
+static void launch_tile_op(uint32_t ntiles, hipStream_t stream, const Operand& A, const Operand& B, int64_t U, int32_t rpt,
+                           int32_t* out_off, int32_t* ob, int32_t* oe, int64_t capacity, int32_t* d_total,
+                           uint32_t* status, void* scratch, uint32_t epoch, int64_t spin) {
+    // operands read as stored (no transform on load): the plain kernel
+    const bool xf = A.xf.mode != XF_NONE || (B.off != nullptr && B.xf.mode != XF_NONE);
+    if (xf)
+        hipLaunchKernelGGL((setop_tile_kernel<OP, MODE, true>), dim3(ntiles), dim3(kBlock), 0, stream, A, B, U, rpt,
+                           ntiles, out_off, ob, oe, capacity, d_total, status, scratch, epoch, spin);
+    else
+        hipLaunchKernelGGL((setop_tile_kernel<OP, MODE, false>), dim3(ntiles), dim3(kBlock), 0, stream, A, B, U, rpt,
+                           ntiles, out_off, ob, oe, capacity, d_total, status, scratch, epoch, spin);
+}
+
+template <int MODE>
+static void launch_tile(int32_t op, uint32_t ntiles, hipStream_t stream, const Operand& A, const Operand& B, int64_t U,
+                        int32_t rpt, int32_t* out_off, int32_t* ob, int32_t* oe, int64_t capacity, int32_t* d_total,
+                        uint32_t* status, void* scratch, uint32_t epoch, int64_t spin) {
+    if (op == 0)
+        launch_tile_op<0, MODE>(ntiles, stream, A, B, U, rpt, out_off, ob, oe, capacity, d_total, status, scratch, epoch,
+                                spin);
+    else if (op == 1)
+        launch_tile_op<1, MODE>(ntiles, stream, A, B, U, rpt, out_off, ob, oe, capacity, d_total, status, scratch, epoch,
+                                spin);
+    else
+        launch_tile_op<2, MODE>(ntiles, stream, A, B, U, rpt, out_off, ob, oe, capacity, d_total, status, scratch, epoch,
+                                spin);
+}
+
 template <int G>
 static int32_t launch_setop_g(int32_t op, const Operand& A, const Operand& B, const int32_t* d_nrows, int64_t nrows_bound,
                               int32_t* out_off, int32_t* ob, int32_t* oe, int64_t capacity, int32_t* d_total,
@@ -658,7 +771,7 @@ int64_t stx_scan_scratch_bytes(int64_t n_bound) {
 }
 
 int64_t stx_setop_scratch_bytes(int64_t nrows_bound, int32_t group) {
-    // the tile path never uses more tiles than the 4-lane row-group path
+    // the tile paths never use more tiles than the 4-lane row-group path
     (void)group;
     return fused_rows_scratch_bytes(nrows_bound, 4) > lookback_bytes(nrows_bound + 1)
                ? fused_rows_scratch_bytes(nrows_bound, 4)
@@ -681,18 +794,22 @@ int32_t stx_setop(int32_t op, const int32_t* a_beg, const int32_t* a_end, const 
     if (tile_path) {
         const int32_t rpt = setop_tile_rows(a_n + (b_off ? b_n : 0), nrows_bound);
         const uint32_t ntiles = (uint32_t)((nrows_bound + rpt) / rpt);
-        if (lookback_bytes(ntiles) > scratch_bytes) return stx_fail(STX_EINVAL, "setop: scratch too small");
+        // group 0: automatic (single pass: the two-pass form was slower at
+        // every size measured); -1: single pass; -2: two passes
+        const bool two_pass = group == -2 || (group == 0 && ntiles >= kTwoPassTiles);
         const int64_t spin = stx_spin_limit();
-        if (op == 0)
-            hipLaunchKernelGGL(setop_tile_kernel<0>, dim3(ntiles), dim3(kBlock), 0, stream, A, B, nrows_bound, rpt, ntiles,
-                               out_off, out_beg, out_end, capacity, d_total, status, scratch, epoch, spin);
-        else if (op == 1)
-            hipLaunchKernelGGL(setop_tile_kernel<1>, dim3(ntiles), dim3(kBlock), 0, stream, A, B, nrows_bound, rpt, ntiles,
-                               out_off, out_beg, out_end, capacity, d_total, status, scratch, epoch, spin);
-        else
-            hipLaunchKernelGGL(setop_tile_kernel<2>, dim3(ntiles), dim3(kBlock), 0, stream, A, B, nrows_bound, rpt, ntiles,
-                               out_off, out_beg, out_end, capacity, d_total, status, scratch, epoch, spin);
-        return stx_launch_status("setop_tile_kernel");
+        if (lookback_bytes(ntiles) > scratch_bytes) return stx_fail(STX_EINVAL, "setop: scratch too small");
+        if (!two_pass) {
+            launch_tile<kFused>(op, ntiles, stream, A, B, nrows_bound, rpt, out_off, out_beg, out_end, capacity, d_total,
+                                status, scratch, epoch, spin);
+            return stx_launch_status("setop_tile_kernel");
+        }
+        launch_tile<kCount>(op, ntiles, stream, A, B, nrows_bound, rpt, out_off, out_beg, out_end, capacity, d_total,
+                            status, scratch, epoch, spin);
+        hipLaunchKernelGGL(prefix_states_kernel, dim3(1), dim3(kPrefixThreads), 0, stream, scratch, ntiles, epoch);
+        launch_tile<kWrite>(op, ntiles, stream, A, B, nrows_bound, rpt, out_off, out_beg, out_end, capacity, d_total,
+                            status, scratch, epoch, spin);
+        return stx_launch_status("setop_tile_kernel(two-pass)");
     }
     const int32_t g = group > 0 ? group : 64;
     if (g >= 64)

@@ -144,9 +144,13 @@ __device__ __forceinline__ unsigned long long state_load(const unsigned long lon
 }
 
 // Called by all 64 lanes of ONE wave of the block.  Publishes `agg` for tile
-// `tile` and returns the exclusive prefix of all earlier tiles.  After
-// `spin_limit` polls the block records STX_FLAG_SPIN in `status` and proceeds
-// (its result is then invalid, but every wave still exits).
+// `tile` and returns the exclusive prefix of all earlier tiles.  Each poll
+// covers a window of 64 * K predecessors (lane l reads tiles
+// pos - l*K - k, k < K, nearest first), so a published prefix is found up to
+// 64 * K tiles back in one round trip.  After `spin_limit` polls the block
+// records STX_FLAG_SPIN in `status` and proceeds (its result is then invalid,
+// but every wave still exits).
+template <int K = 1>
 __device__ __forceinline__ uint32_t lookback_exclusive(LookbackScratch s, uint32_t tile, uint32_t agg, uint32_t epoch,
                                                        uint32_t* status, int64_t spin_limit) {
     const int lane = lane_id();
@@ -159,20 +163,38 @@ __device__ __forceinline__ uint32_t lookback_exclusive(LookbackScratch s, uint32
     int64_t pos = (int64_t)tile - 1;  // nearest predecessor not yet folded in
     int64_t spins = 0;
     while (pos >= 0) {
-        int64_t p = pos - lane;
-        uint32_t flag = 2u, val = 0u;
-        if (p >= 0) {
-            unsigned long long w = state_load(&s.state[p]);
-            uint32_t hi = (uint32_t)(w >> 32);
-            flag = (hi >> 2) == epoch ? (hi & 3u) : 0u;
-            val = (uint32_t)w;
+        // this lane's K predecessors, nearest first
+        uint32_t flag[K], val[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int64_t p = pos - (int64_t)lane * K - k;
+            flag[k] = 2u;  // before tile 0: an empty prefix
+            val[k] = 0u;
+            if (p >= 0) {
+                const unsigned long long w = state_load(&s.state[p]);
+                const uint32_t hi = (uint32_t)(w >> 32);
+                flag[k] = (hi >> 2) == epoch ? (hi & 3u) : 0u;
+                val[k] = (uint32_t)w;
+            }
         }
-        uint64_t not_ready = __ballot(flag == 0u);
-        uint64_t is_prefix = __ballot(flag == 2u);
-        // lanes up to (and including) the first prefix lane are usable only if
-        // none of them is still unpublished
-        int first_prefix = is_prefix ? __ffsll((long long)is_prefix) - 1 : 64;
-        uint64_t upto = first_prefix >= 63 ? ~0ull : ((1ull << (first_prefix + 1)) - 1ull);
+        // within the lane: first prefix, readiness before it, partial sum
+        int kp = K;
+        bool blocked = false;
+        uint32_t part = 0;
+#pragma unroll
+        for (int k = K - 1; k >= 0; --k)
+            if (flag[k] == 2u) kp = k;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            if (k <= kp && k < K) {
+                if (flag[k] == 0u) blocked = true;
+                part += val[k];
+            }
+        }
+        const uint64_t has_prefix = __ballot(kp < K);
+        const uint64_t not_ready = __ballot(blocked);
+        const int first_prefix = has_prefix ? __ffsll((long long)has_prefix) - 1 : 64;
+        const uint64_t upto = first_prefix >= 63 ? ~0ull : ((1ull << (first_prefix + 1)) - 1ull);
         if (not_ready & upto) {
             if (++spins > spin_limit) {
                 if (lane == 0) atomicOr(status, STX_FLAG_SPIN);
@@ -181,12 +203,12 @@ __device__ __forceinline__ uint32_t lookback_exclusive(LookbackScratch s, uint32
             __builtin_amdgcn_s_sleep(1);
             continue;
         }
-        uint32_t contrib = (lane <= first_prefix && p >= 0) ? val : 0u;
+        uint32_t contrib = lane <= first_prefix ? part : 0u;
 #pragma unroll
         for (int d = 32; d >= 1; d >>= 1) contrib += __shfl_xor(contrib, d, 64);
         excl += contrib;
         if (first_prefix < 64) break;
-        pos -= 64;
+        pos -= 64 * K;
     }
     if (lane == 0) state_store(&s.state[tile], epoch, 2u, excl + agg);
     return excl;
