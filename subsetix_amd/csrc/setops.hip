@@ -205,8 +205,10 @@ struct SetopRow {
 // the block, the tile offset comes from the decoupled look-back, and the
 // intervals are written in tile order (contiguous).  A tile whose rows hold
 // more than PCAP intervals falls back to one wavefront per row.
+constexpr int kTB = 512;              // threads of a tile block
+constexpr int kTW = kTB / STX_WAVE;   // its waves
 constexpr int kVT = 8;
-constexpr int kPCAP = kBlock * kVT;
+constexpr int kPCAP = kTB * kVT;
 constexpr int kMaxRPT = 256;
 
 struct SegMax {
@@ -222,7 +224,7 @@ __device__ __forceinline__ SegMax seg_combine(SegMax a, SegMax b) {  // a before
 }
 
 // Block-wide exclusive segmented max-scan (one element per thread).
-__device__ __forceinline__ SegMax block_seg_max_exclusive(SegMax x, SegMax* lds /* >= kWaves */) {
+__device__ __forceinline__ SegMax block_seg_max_exclusive(SegMax x, SegMax* lds /* >= kTW */) {
     const int lane = lane_id(), wave = wave_id();
     SegMax inc = x;
 #pragma unroll
@@ -266,7 +268,7 @@ __device__ __forceinline__ bool key_a_first(int32_t akb, int32_t bkb, int32_t bk
 constexpr int kFused = 0, kCount = 1, kWrite = 2;
 
 template <int OP, int MODE, bool XF>
-__global__ __launch_bounds__(kBlock) void setop_tile_kernel(Operand A, Operand B, int64_t U, int32_t RPT, uint32_t ntiles,
+__global__ __launch_bounds__(kTB) void setop_tile_kernel(Operand A, Operand B, int64_t U, int32_t RPT, uint32_t ntiles,
                                                             int32_t* __restrict__ out_off, int32_t* __restrict__ ob,
                                                             int32_t* __restrict__ oe, int64_t capacity,
                                                             int32_t* __restrict__ d_total, uint32_t* status,
@@ -277,10 +279,10 @@ __global__ __launch_bounds__(kBlock) void setop_tile_kernel(Operand A, Operand B
     __shared__ int32_t s_bS[kMaxRPT + 1];
     // per thread: emissions before its first position, and its emission mask
     // (fallback path: per-row counts, then their exclusive scan)
-    __shared__ int32_t s_tbase[kBlock + 1];
-    __shared__ uint8_t s_tmask[kBlock];
+    __shared__ int32_t s_tbase[kTB + 1];
+    __shared__ uint8_t s_tmask[kTB];
     __shared__ int32_t s_scan[17];
-    __shared__ SegMax s_seg[kWaves];
+    __shared__ SegMax s_seg[kTW];
     __shared__ uint32_t s_excl;
 
     LookbackScratch lb = lookback_scratch(scratch, ntiles);
@@ -293,7 +295,7 @@ __global__ __launch_bounds__(kBlock) void setop_tile_kernel(Operand A, Operand B
     // exists; the trailing tile (nr == 0 when RPT divides U) then sees P = 0.
     const int32_t a_base = A.off[row0];
     const int32_t b_base = B.off ? B.off[row0] : 0;
-    for (int r = tid; r <= nr; r += kBlock) {
+    for (int r = tid; r <= nr; r += kTB) {
         int32_t as = A.off[row0 + r] - a_base;
         int32_t bs = B.off ? B.off[row0 + r] - b_base : 0;
         s_aS[r] = as;
@@ -313,11 +315,11 @@ __global__ __launch_bounds__(kBlock) void setop_tile_kernel(Operand A, Operand B
 
     if (fits) {
         if (XF) {
-            for (int k = tid; k < nA; k += kBlock) s_k[k] = make_int2(keyb(A, a_base + k), keye(A, a_base + k));
-            for (int k = tid; k < nB; k += kBlock) s_k[nA + k] = make_int2(keyb(B, b_base + k), keye(B, b_base + k));
+            for (int k = tid; k < nA; k += kTB) s_k[k] = make_int2(keyb(A, a_base + k), keye(A, a_base + k));
+            for (int k = tid; k < nB; k += kTB) s_k[nA + k] = make_int2(keyb(B, b_base + k), keye(B, b_base + k));
         } else {
-            for (int k = tid; k < nA; k += kBlock) s_k[k] = make_int2(A.beg[a_base + k], A.end[a_base + k]);
-            for (int k = tid; k < nB; k += kBlock) s_k[nA + k] = make_int2(B.beg[b_base + k], B.end[b_base + k]);
+            for (int k = tid; k < nA; k += kTB) s_k[k] = make_int2(A.beg[a_base + k], A.end[a_base + k]);
+            for (int k = tid; k < nB; k += kTB) s_k[nA + k] = make_int2(B.beg[b_base + k], B.end[b_base + k]);
         }
         __syncthreads();
         const int32_t p0 = tid * kVT;
@@ -343,91 +345,98 @@ __global__ __launch_bounds__(kBlock) void setop_tile_kernel(Operand A, Operand B
                 else chi = mid;
             }
             int32_t i = clo, j = l - clo;
-            // the walk keeps the current and previous element of each operand
-            // in registers: one LDS read per consumed element
-            int2 curA = lds_or_none(s_k, aS + i, i < na);
-            int2 curB = lds_or_none(s_k, bS + j, j < nb);
-            int2 prevA = lds_or_none(s_k, aS + i - 1, i >= 1);
-            int2 prevB = lds_or_none(s_k, bS + j - 1, j >= 1);
+            // The walk keeps the current and previous element of each operand
+            // in registers (as scalars: selecting between int2 values makes
+            // the compiler select between their addresses) and is branch-free:
+            // each position picks its element by value, computes the emission
+            // of both cases, and reads only the next element of the side it
+            // consumes (one LDS read per position).
+            int32_t cAx, cAy, cBx, cBy, pAx, pAy, pBx, pBy;
+            {
+                const int2 t0 = s_k[i < na ? aS + i : 0];
+                const int2 t1 = s_k[j < nb ? bS + j : 0];
+                const int2 t2 = s_k[i >= 1 ? aS + i - 1 : 0];
+                const int2 t3 = s_k[j >= 1 ? bS + j - 1 : 0];
+                cAx = i < na ? t0.x : 0;
+                cAy = i < na ? t0.y : INT_MIN;
+                cBx = j < nb ? t1.x : 0;
+                cBy = j < nb ? t1.y : INT_MIN;
+                pAx = i >= 1 ? t2.x : 0;
+                pAy = i >= 1 ? t2.y : INT_MIN;
+                pBx = j >= 1 ? t3.x : 0;
+                pBy = j >= 1 ? t3.y : INT_MIN;
+            }
 #pragma unroll
             for (int v = 0; v < kVT; ++v) {
                 const int32_t p = p0 + v;
                 rb[v] = 0;
                 re[v] = INT_MIN;
                 if (p < P) {
-                    while (p >= rowend) {  // next non-empty row
-                        ++r;
-                        aS = s_aS[r];
-                        na = s_aS[r + 1] - aS;
-                        bS = nA + s_bS[r];
-                        nb = s_bS[r + 1] - s_bS[r];
-                        rs = rowend;
-                        rowend = s_aS[r + 1] + s_bS[r + 1];
+                    if (p >= rowend) {
+                        do {  // next non-empty row
+                            ++r;
+                            aS = s_aS[r];
+                            na = s_aS[r + 1] - aS;
+                            bS = nA + s_bS[r];
+                            nb = s_bS[r + 1] - s_bS[r];
+                            rs = rowend;
+                            rowend = s_aS[r + 1] + s_bS[r + 1];
+                        } while (p >= rowend);
                         i = 0;
                         j = 0;
-                        curA = lds_or_none(s_k, aS, na > 0);
-                        curB = lds_or_none(s_k, bS, nb > 0);
+                        const int2 t0 = s_k[na > 0 ? aS : 0];
+                        const int2 t1 = s_k[nb > 0 ? bS : 0];
+                        cAx = na > 0 ? t0.x : 0;
+                        cAy = na > 0 ? t0.y : INT_MIN;
+                        cBx = nb > 0 ? t1.x : 0;
+                        cBy = nb > 0 ? t1.y : INT_MIN;
                     }
-                    if (p == rs) rowstart_mask |= 1u << v;
-                    if (p == rowend - 1) rowend_mask |= 1u << v;
-                    const bool isA = (j >= nb) || (i < na && key_a_first<OP>(curA.x, curB.x, curB.y));
+                    rowstart_mask |= (p == rs ? 1u : 0u) << v;
+                    rowend_mask |= (p == rowend - 1 ? 1u : 0u) << v;
+                    const bool isA = (j >= nb) || (i < na && key_a_first<OP>(cAx, cBx, cBy));
+                    // next element of the consumed side
+                    const bool nvalid = isA ? (i + 1 < na) : (j + 1 < nb);
+                    const int2 nraw = s_k[nvalid ? (isA ? aS + i + 1 : bS + j + 1) : 0];
+                    const int32_t nxx = nvalid ? nraw.x : 0;
+                    const int32_t nxy = nvalid ? nraw.y : INT_MIN;
                     if (OP == 0) {
-                        const int2 e = isA ? curA : curB;
+                        const int32_t ex = isA ? cAx : cBx;
+                        const int32_t ey = isA ? cAy : cBy;
                         const bool clip = XF && (isA ? A.xf.mode == XF_CLIP : B.xf.mode == XF_CLIP);
-                        if (!(clip && e.x >= e.y)) alive_mask |= 1u << v;
-                        rb[v] = e.x;
-                        re[v] = e.y;
+                        alive_mask |= ((clip && ex >= ey) ? 0u : 1u) << v;
+                        rb[v] = ex;
+                        re[v] = ey;
                     } else if (OP == 1) {
-                        if (isA) {
-                            const int32_t ab = curA.x, ae = curA.y;
-                            int32_t ce = INT_MIN;
-                            if (j >= 1 && prevB.y > ab) ce = prevB.y;
-                            else if (j < nb && curB.x == ab) ce = curB.y;
-                            if (ce != INT_MIN) { emit_mask |= 1u << v; rb[v] = ab; re[v] = ae < ce ? ae : ce; }
-                        } else {
-                            const int32_t bb = curB.x, be = curB.y;
-                            if (i >= 1 && prevA.x < bb && bb < prevA.y) {
-                                emit_mask |= 1u << v;
-                                rb[v] = bb;
-                                re[v] = be < prevA.y ? be : prevA.y;
-                            }
-                        }
+                        // A element: clipped by the B element covering its begin
+                        const int32_t ceA = (j >= 1 && pBy > cAx) ? pBy : ((j < nb && cBx == cAx) ? cBy : INT_MIN);
+                        const bool emA = ceA != INT_MIN;
+                        // B element: inside the previous A element
+                        const bool emB = i >= 1 && pAx < cBx && cBx < pAy;
+                        emit_mask |= ((isA ? emA : emB) ? 1u : 0u) << v;
+                        rb[v] = isA ? cAx : cBx;
+                        re[v] = isA ? (cAy < ceA ? cAy : ceA) : (cBy < pAy ? cBy : pAy);
                     } else {
-                        if (isA) {
-                            const int32_t ab = curA.x, ae = curA.y;
-                            const bool covered = j < nb && curB.x <= ab;
-                            if (!covered) {
-                                int32_t pe = ae;
-                                if (j < nb) pe = pe < curB.x ? pe : curB.x;
-                                emit_mask |= 1u << v;
-                                rb[v] = ab;
-                                re[v] = pe;
-                            }
-                        } else {
-                            const int32_t be = curB.y;
-                            if (i >= 1) {
-                                const int32_t pae = prevA.y;
-                                const int2 nxt = lds_or_none(s_k, bS + j + 1, j + 1 < nb);
-                                const bool open = (j + 1 >= nb) || nxt.x > be;
-                                if (be < pae && open) {
-                                    int32_t pe = pae;
-                                    if (j + 1 < nb) pe = pe < nxt.x ? pe : nxt.x;
-                                    emit_mask |= 1u << v;
-                                    rb[v] = be;
-                                    re[v] = pe;
-                                }
-                            }
-                        }
+                        // A element not covered at its begin: runs to the next B begin
+                        const bool emA = !(j < nb && cBx <= cAx);
+                        const int32_t peA = (j < nb && cBx < cAy) ? cBx : cAy;
+                        // B element ending inside the previous A element
+                        const bool openB = (j + 1 >= nb) || nxx > cBy;
+                        const bool emB = i >= 1 && cBy < pAy && openB;
+                        const int32_t peB = (j + 1 < nb && nxx < pAy) ? nxx : pAy;
+                        emit_mask |= ((isA ? emA : emB) ? 1u : 0u) << v;
+                        rb[v] = isA ? cAx : cBy;
+                        re[v] = isA ? peA : peB;
                     }
-                    if (isA) {
-                        prevA = curA;
-                        ++i;
-                        curA = lds_or_none(s_k, aS + i, i < na);
-                    } else {
-                        prevB = curB;
-                        ++j;
-                        curB = lds_or_none(s_k, bS + j, j < nb);
-                    }
+                    pAx = isA ? cAx : pAx;
+                    pAy = isA ? cAy : pAy;
+                    pBx = isA ? pBx : cBx;
+                    pBy = isA ? pBy : cBy;
+                    cAx = isA ? nxx : cAx;
+                    cAy = isA ? nxy : cAy;
+                    cBx = isA ? cBx : nxx;
+                    cBy = isA ? cBy : nxy;
+                    i += isA ? 1 : 0;
+                    j += isA ? 0 : 1;
                 }
             }
         }
@@ -456,7 +465,7 @@ __global__ __launch_bounds__(kBlock) void setop_tile_kernel(Operand A, Operand B
     } else {
         // fallback: one wavefront per row, counts into s_tbase[r]
         const int wave = wave_id();
-        for (int r = wave; r < nr; r += kWaves) {
+        for (int r = wave; r < nr; r += kTW) {
             int32_t a0, na, b0, nb;
             row_extent(A, row0 + r, a0, na);
             row_extent(B, row0 + r, b0, nb);
@@ -466,7 +475,7 @@ __global__ __launch_bounds__(kBlock) void setop_tile_kernel(Operand A, Operand B
         __syncthreads();
         int32_t v = tid < nr ? s_tbase[tid] : 0;
         int32_t ex = 0;
-        // nr <= kMaxRPT == kBlock
+        // nr <= kMaxRPT <= kTB
         ex = block_exclusive_scan<int32_t>(v, s_scan, &tile_total);
         __syncthreads();
         if (tid < nr) s_tbase[tid] = ex;
@@ -484,7 +493,7 @@ __global__ __launch_bounds__(kBlock) void setop_tile_kernel(Operand A, Operand B
     const int64_t excl =
         MODE == kFused ? (int64_t)s_excl : (tile == 0 ? 0 : (int64_t)(uint32_t)state_load(&lb.state[tile - 1]));
     // row offsets
-    for (int r = tid; r <= nr; r += kBlock) {
+    for (int r = tid; r <= nr; r += kTB) {
         if (r == nr && row0 + nr != U) break;
         int32_t local;
         if (fits) {
@@ -541,7 +550,7 @@ __global__ __launch_bounds__(kBlock) void setop_tile_kernel(Operand A, Operand B
         }
     } else {
         const int wave = wave_id();
-        for (int r = wave; r < nr; r += kWaves) {
+        for (int r = wave; r < nr; r += kTW) {
             int32_t a0, na, b0, nb;
             row_extent(A, row0 + r, a0, na);
             row_extent(B, row0 + r, b0, nb);
@@ -720,10 +729,10 @@ static void launch_tile_op(uint32_t ntiles, hipStream_t stream, const Operand& A
     // operands read as stored (no transform on load): the plain kernel
     const bool xf = A.xf.mode != XF_NONE || (B.off != nullptr && B.xf.mode != XF_NONE);
     if (xf)
-        hipLaunchKernelGGL((setop_tile_kernel<OP, MODE, true>), dim3(ntiles), dim3(kBlock), 0, stream, A, B, U, rpt,
+        hipLaunchKernelGGL((setop_tile_kernel<OP, MODE, true>), dim3(ntiles), dim3(kTB), 0, stream, A, B, U, rpt,
                            ntiles, out_off, ob, oe, capacity, d_total, status, scratch, epoch, spin);
     else
-        hipLaunchKernelGGL((setop_tile_kernel<OP, MODE, false>), dim3(ntiles), dim3(kBlock), 0, stream, A, B, U, rpt,
+        hipLaunchKernelGGL((setop_tile_kernel<OP, MODE, false>), dim3(ntiles), dim3(kTB), 0, stream, A, B, U, rpt,
                            ntiles, out_off, ob, oe, capacity, d_total, status, scratch, epoch, spin);
 }
 
