@@ -9,6 +9,8 @@ host synchronisation inside the step.  value = input intervals processed per
 second (the reference's ns-per-input-interval convention, inverted).
 
 Also reported beside the headline (each in its own object):
+  copy           1 GiB device copy: the achievable-bandwidth reference and the
+                 calibration kernel for the HBM byte counters
   plain_form     one pair through the reference-shaped evaluate() call
   morphology     dilate / erode, square SE radius 1, 2, 4 (one pair)
   large_stencil  8192^2 fine / 4096^2 coarse composite, 500 steps, FP32 vs FP64
@@ -35,7 +37,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-SECTIONS = ("morphology", "large_stencil", "batched_amr", "plain_amr", "serve")
+SECTIONS = ("copy", "morphology", "large_stencil", "batched_amr", "plain_amr", "serve")
 
 
 def parse():
@@ -54,6 +56,7 @@ def parse():
     p.add_argument("--amr-steps", type=int, default=200)
     p.add_argument("--serve-rates", default="100,1000,10000", help="Poisson arrival rates (requests/s) for the serve section")
     p.add_argument("--serve-requests", type=int, default=64)
+    p.add_argument("--plain-reps", type=int, default=50, help="repetitions of the plain single-pair form")
     return p.parse_args()
 
 
@@ -102,13 +105,15 @@ def main():
     head = perf.setalgebra_batched(args.batch, args.steps, args.warmup, dev, group=args.group)
     log(f"set algebra ({time.time() - t0:.1f}s): kernel ms {head['kernel_ms']} GB/s {head['kernel_gbs']}")
     pair = head.pop("host_pair")
-    plain = perf.setalgebra_plain(pair, 50, dev)
+    plain = perf.setalgebra_plain(pair, args.plain_reps, dev)
     achieved = head["kernel_gbs"]["union"]
 
     extra = {}
     for s in sections:
         t0 = time.time()
-        if s == "morphology":
+        if s == "copy":
+            extra[s] = perf.copy_bandwidth(dev)
+        elif s == "morphology":
             extra[s] = perf.morphology(pair, 10, dev)
         elif s == "large_stencil":
             extra[s] = perf.large_stencil(args.stencil_steps, dev)

@@ -96,9 +96,16 @@ class CompositeTwoLevel:
         from ..interval_field import cell_offsets
         from ..interval_stencil import stencil_plan
 
+        from ..levels import transfer_plan
+
         for t in (self.ghost, self.refine):
             if t.interval_count:
                 cell_offsets(t)
+        if self.ratio == 2:
+            if self.ghost.interval_count:
+                transfer_plan(0, self.ghost, self.fine_field, self.coarse)
+            if self.refine.interval_count:
+                transfer_plan(1, self.refine, self.coarse, self.fine_field)
         stencil_plan(self.coarse_set, self._c_co, self.width)
         stencil_plan(self.ext, self._f_co, self.width * self.ratio)
 

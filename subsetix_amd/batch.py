@@ -36,7 +36,7 @@ from ._lib import _I32, _I64, _P, check, context, ptr
 from .expressions import IntervalSet, binary_op
 from .interval_field import IntervalField, cell_offsets, create_interval_field
 from .interval_stencil import build_stencil_plan
-from .levels import gradient_tag, prolong_into, restrict_into, upwind_batched
+from .levels import gradient_tag, prolong_into, restrict_into, transfer_plan, upwind_batched
 from .morphology import dilate_interval_set
 from .multilevel import prolong_set, restrict_set
 
@@ -196,8 +196,14 @@ class BatchedAMR:
         self.row_inst = [self._row_inst(self.ext[lv], lv) for lv in range(L)]
         for t in self.ghosts[1:] + self.refines:
             cell_offsets(t)
-        # stencil tile plans (per-interval instance and width), built before
-        # any graph capture
+        # transfer and stencil tables, built before any graph capture
+        if R == 2:
+            fl = self._fields(self.cur)
+            for lv in range(1, L):
+                if self.ghosts[lv].interval_count:
+                    transfer_plan(0, self.ghosts[lv], fl[lv], fl[lv - 1])
+                if self.refines[lv - 1].interval_count:
+                    transfer_plan(1, self.refines[lv - 1], fl[lv - 1], fl[lv])
         self.plans = [build_stencil_plan(self.ext[lv], self.f[lv].interval_cell_offsets, 0, self.row_inst[lv],
                                          self.inst_w[lv]) for lv in range(L)]
         self.graph = None

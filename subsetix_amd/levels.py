@@ -21,6 +21,13 @@ _lib.register("stx_transfer", _I32, [_I32, _I32, _I32, _I32, _I32, _P, _P, _P, _
 _lib.register("stx_grad_tag", _I32, [_I32, _I32, _P, _P, _I64, _P, _P, _P, _P, _F64, _P, _P, _P, _I64, _P, _P, _P,
                                      _I64, _I32, _P])
 _lib.register("stx_upwind_batched", _I32, [_I32, _I32, _P, _P, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P])
+_GEOM7 = [_P, _P, _P, _P, _P, _I64]
+_lib.register("stx_transfer_tile_cells", _I64, [])
+_lib.register("stx_transfer_seg_count", _I32, [_I32] + _GEOM7 + [_P, _I64] + _GEOM7 + _GEOM7
+              + [_P, _P, _P, _P, _P, _I64, _I32, _P])
+_lib.register("stx_transfer_seg_build", _I32, [_I32] + _GEOM7 + [_P, _I64] + _GEOM7 + _GEOM7
+              + [_P, _I64, _P, _I64, _P, _I64, _P])
+_lib.register("stx_transfer_seg", _I32, [_I32, _I32, _I32, _P, _I64, _I64, _P, _P, _I64, _P, _P])
 
 
 def _geom(s: IntervalSet, co: torch.Tensor):
@@ -40,6 +47,59 @@ def _dtype_code(t: torch.Tensor) -> int:
     raise TypeError("transfers need float32 or float64 fields")
 
 
+def transfer_plan(mode: int, target: IntervalSet, dst: IntervalField, src: IntervalField):
+    """Segment table of a ratio-2 transfer onto ``target`` (cached on the
+    target set, keyed by the destination and source geometries)."""
+    cache = target._meta.setdefault("transfer_plans", {})
+    key = (mode, id(dst.interval_set), id(src.interval_set), id(dst.interval_cell_offsets),
+           id(src.interval_cell_offsets))
+    hit = cache.get(key)
+    if hit is not None:
+        return hit[0]
+    dev = target.device
+    ctx = context(dev)
+    tco = cell_offsets(target)
+    n = target.interval_count
+    cells = int(tco[-1].item()) if n else 0
+    tile = int(_lib.lib().stx_transfer_tile_cells())
+    ntiles = (cells + tile - 1) // tile
+    ord_ = target.interval_row_ordinals() if n else _core.empty_i32(1, dev)
+    counts = _core.empty_i32(max(n, 1), dev)
+    seg_off = _core.empty_i32(n + 1, dev)
+    sp, sbytes, ep = ctx.scratch(_lib.lib().stx_scan_scratch_bytes(n))
+    tg = _geom(target, tco)
+    dg = _geom(dst.interval_set, dst.interval_cell_offsets)
+    sgm = _geom(src.interval_set, src.interval_cell_offsets)
+    check(_lib.lib().stx_transfer_seg_count(mode, *tg, ptr(ord_), n, *dg, *sgm, ptr(counts), ptr(seg_off),
+                                            ctx.slot(0), ctx.status_ptr, sp, sbytes, ep, ctx.stream),
+          "transfer_seg_count")
+    (nseg,) = _core.read_counts(ctx, ctx.words[1])
+    segs = _core.empty_i32(8 * max(nseg, 1), dev)
+    plan = _core.empty_i32(2 * max(ntiles, 1), dev)
+    check(_lib.lib().stx_transfer_seg_build(mode, *tg, ptr(ord_), n, *dg, *sgm, ptr(seg_off), nseg, ptr(segs), cells,
+                                            ptr(plan), ntiles, ctx.stream), "transfer_seg_build")
+    out = (plan, ntiles, cells, segs)
+    cache[key] = (out, dst.interval_set, src.interval_set, dst.interval_cell_offsets, src.interval_cell_offsets)
+    return out
+
+
+_TRANSFER_SCHEDULE = "segments"
+
+
+def set_transfer_schedule(name: str) -> None:
+    """'segments' (segment tables, ratio 2; default) or 'rows' (lane group
+    per target row, any ratio)."""
+    global _TRANSFER_SCHEDULE
+    if name not in ("segments", "rows"):
+        raise ValueError("schedule must be 'segments' or 'rows'")
+    _TRANSFER_SCHEDULE = name
+
+
+def _seg_ok(dst: IntervalField, src: IntervalField, ratio: int) -> bool:
+    return (_TRANSFER_SCHEDULE == "segments" and ratio == 2
+            and src.values.numel() * src.values.element_size() < (1 << 32) - 256)
+
+
 def prolong_into(target: IntervalSet, dst: IntervalField, src: IntervalField, ratio: int) -> None:
     """dst(y, x) = src(y // ratio, x // ratio) for every cell of ``target``
     (cells of target must lie in dst's geometry)."""
@@ -48,6 +108,12 @@ def prolong_into(target: IntervalSet, dst: IntervalField, src: IntervalField, ra
     if dst.values.dtype != src.values.dtype:
         raise TypeError("source and destination fields must share a dtype")
     ctx = context(dst.values.device)
+    if _seg_ok(dst, src, ratio):
+        plan, ntiles, cells, segs = transfer_plan(0, target, dst, src)
+        check(_lib.lib().stx_transfer_seg(0, _dtype_code(dst.values), 0, ptr(plan), ntiles, cells, ptr(segs),
+                                          ptr(src.values), int(src.values.numel()), ptr(dst.values), ctx.stream),
+              "prolong_into")
+        return
     tco = cell_offsets(target)
     n = int(dst.values.numel())
     check(_lib.lib().stx_transfer(0, _dtype_code(dst.values), 0, _group(n, target.row_count), int(ratio),
@@ -65,8 +131,14 @@ def restrict_into(target: IntervalSet, dst: IntervalField, src: IntervalField, r
     if dst.values.dtype != src.values.dtype:
         raise TypeError("source and destination fields must share a dtype")
     ctx = context(dst.values.device)
-    tco = cell_offsets(target)
     acc64 = 1 if accumulate == "f64" else 0
+    if _seg_ok(dst, src, ratio):
+        plan, ntiles, cells, segs = transfer_plan(1, target, dst, src)
+        check(_lib.lib().stx_transfer_seg(1, _dtype_code(dst.values), acc64, ptr(plan), ntiles, cells, ptr(segs),
+                                          ptr(src.values), int(src.values.numel()), ptr(dst.values), ctx.stream),
+              "restrict_into")
+        return
+    tco = cell_offsets(target)
     check(_lib.lib().stx_transfer(1, _dtype_code(dst.values), acc64, _group(int(src.values.numel()), target.row_count),
                                   int(ratio), *_geom(target, tco), *_geom(dst.interval_set, dst.interval_cell_offsets),
                                   *_geom(src.interval_set, src.interval_cell_offsets), ptr(src.values),

@@ -367,3 +367,21 @@ def serve(rates, count: int, dev, seed: int = 5) -> Dict:
             "throughput_ratio": srv["throughput_req_per_s"] / max(plain["throughput_req_per_s"], 1e-9),
         }
     return out
+
+
+def copy_bandwidth(dev, nbytes: int = 1 << 30, reps: int = 10) -> Dict:
+    """Device-to-device copy of ``nbytes`` (the calibration kernel for HBM
+    byte counters and the achievable-bandwidth reference)."""
+    src = torch.empty(nbytes // 4, dtype=torch.float32, device=dev).fill_(1.0)
+    dst = torch.empty_like(src)
+    for _ in range(2):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    s, e = _events()
+    s.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e.record()
+    torch.cuda.synchronize()
+    ms = s.elapsed_time(e) / reps
+    return {"bytes_copied": nbytes, "ms": ms, "gbs_read_plus_write": 2 * nbytes / (ms * 1e-3) / 1e9}

@@ -40,17 +40,35 @@ def _field(sx, s, vals, dtype):
     return f
 
 
+@pytest.mark.parametrize("schedule", ["segments", "rows"])
 @pytest.mark.parametrize("dt", ["f32", "f64"])
-def test_prolong_restrict_into(sx, dt):
+@pytest.mark.parametrize("touching", [False, True])
+def test_prolong_restrict_into(sx, dt, schedule, touching):
+    """Both transfer schedules against a per-cell model: unaligned fine sets
+    (children partly missing), touching intervals, sparse targets."""
+    import torch
+
+    from subsetix_amd import levels
+    from subsetix_amd.levels import prolong_into, restrict_into
+
+    levels.set_transfer_schedule(schedule)
+    try:
+        _prolong_restrict_case(sx, dt, touching)
+    finally:
+        levels.set_transfer_schedule("segments")
+
+
+def _prolong_restrict_case(sx, dt, touching):
     import torch
 
     from subsetix_amd.levels import prolong_into, restrict_into
 
+    print("transfer case", dt, touching, flush=True)
     tdt = torch.float32 if dt == "f32" else torch.float64
     npdt = np.float32 if dt == "f32" else np.float64
-    rng = np.random.default_rng(1)
-    coarse = random_canonical(rng, 30, 60, mean_n=3, rows=np.arange(30))
-    fine = random_canonical(rng, 60, 120, mean_n=5, rows=np.arange(60))
+    rng = np.random.default_rng(1 + int(touching))
+    coarse = random_canonical(rng, 30, 60, mean_n=3, rows=np.arange(30), touching=touching)
+    fine = random_canonical(rng, 60, 120, mean_n=5, rows=np.arange(60), touching=touching)
     target_f = random_canonical(rng, 60, 120, mean_n=4, rows=np.arange(60))
     target_f = ox.intersection(target_f, fine)
     cv = rng.random(int(ox.cell_offsets(coarse)[-1])).astype(npdt)
