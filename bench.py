@@ -64,6 +64,19 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def measured_traffic(batch):
+    """HBM bytes per fused-union launch from the committed rocprofv3 summary
+    of this workload (FETCH_SIZE / WRITE_SIZE, copy-kernel calibrated), or
+    None when the summary does not cover the configuration run."""
+    path = os.path.join(ROOT, "profiles", "setop_union_batched128.md")
+    if batch != 128 or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":")[1])
+    return None
+
+
 def cpu_baseline(pair, seconds):
     from oracle import oracle as ox
 
@@ -155,7 +168,8 @@ def main():
                        "ms_per_op": plain["ms_per_op"]},
         "batched_vs_plain": head["interval_ops_per_s"] / plain["interval_ops_per_s"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "setop fused union",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.batch),
+                     "kernel": "setop fused union",
                      "algorithmic_bytes": head["algorithmic_bytes"]["union"]},
         "cpu_baseline": cpu,
     }
