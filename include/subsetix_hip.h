@@ -223,6 +223,30 @@ int32_t stx_transfer(int32_t mode, int32_t dtype, int32_t acc64, int32_t group, 
                      const int32_t* d_co, int64_t d_R, const int32_t* s_rows, const int32_t* s_off, const int32_t* s_beg,
                      const int32_t* s_end, const int32_t* s_co, int64_t s_R, const void* src, void* dst,
                      hipStream_t stream);
+/* Segment schedule of the ratio-2 stx_transfer (same chains,
+ * fields.py:570-612): stx_transfer_seg_count (per target interval counts,
+ * their scan into seg_off[tN+1], total in *d_total), stx_transfer_seg_build
+ * (records, plan = 2 int32 per tile of stx_transfer_tile_cells() target
+ * cells), then stx_transfer_seg per step.  t_ord: row index of every target
+ * interval; CT: target cell count; src_cells: source field size (cells the
+ * source geometry does not hold read as 0, which equals skipping them). */
+int64_t stx_transfer_tile_cells(void);
+int32_t stx_transfer_seg_count(int32_t mode, const int32_t* t_rows, const int32_t* t_off, const int32_t* t_beg,
+                               const int32_t* t_end, const int32_t* t_co, int64_t t_R, const int32_t* t_ord,
+                               int64_t tN, const int32_t* d_rows, const int32_t* d_off, const int32_t* d_beg,
+                               const int32_t* d_end, const int32_t* d_co, int64_t d_R, const int32_t* s_rows,
+                               const int32_t* s_off, const int32_t* s_beg, const int32_t* s_end, const int32_t* s_co,
+                               int64_t s_R, int32_t* counts, int32_t* seg_off, int32_t* d_total, uint32_t* status,
+                               void* scratch, int64_t scratch_bytes, uint32_t epoch, hipStream_t stream);
+int32_t stx_transfer_seg_build(int32_t mode, const int32_t* t_rows, const int32_t* t_off, const int32_t* t_beg,
+                               const int32_t* t_end, const int32_t* t_co, int64_t t_R, const int32_t* t_ord,
+                               int64_t tN, const int32_t* d_rows, const int32_t* d_off, const int32_t* d_beg,
+                               const int32_t* d_end, const int32_t* d_co, int64_t d_R, const int32_t* s_rows,
+                               const int32_t* s_off, const int32_t* s_beg, const int32_t* s_end, const int32_t* s_co,
+                               int64_t s_R, const int32_t* seg_off, int64_t nseg, int32_t* segs, int64_t CT,
+                               int32_t* plan, int64_t ntiles, hipStream_t stream);
+int32_t stx_transfer_seg(int32_t mode, int32_t dtype, int32_t acc64, const int32_t* plan, int64_t ntiles, int64_t CT,
+                         const int32_t* segs, const void* src, int64_t src_cells, void* dst, hipStream_t stream);
 
 /* Instance of every row of a stack (last i with bases[i] <= rows[k]). */
 int32_t stx_row_instance(const int32_t* rows, int64_t R, const int32_t* bases, int32_t ninst, int32_t* out,

@@ -35,9 +35,26 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
 
 
+def test_every_exported_symbol_is_declared():
+    import subprocess
+
+    from subsetix_amd import _lib
+
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if " T stx_" in line}
+    undeclared = sorted(exported - set(_declared()))
+    assert not undeclared, undeclared
+
+
 def test_python_bindings_match_header():
     import subsetix_amd  # noqa: F401  (registers every binding)
     import subsetix_amd.amr2  # noqa: F401
+    import subsetix_amd.amr2.composite  # noqa: F401
+    import subsetix_amd.batch  # noqa: F401
+    import subsetix_amd.interval_stencil  # noqa: F401
+    import subsetix_amd.levels  # noqa: F401
+    import subsetix_amd.serve  # noqa: F401
     from subsetix_amd import _lib
 
     decl = _declared()
