@@ -167,6 +167,7 @@ def main():
         "plain_form": {"value": plain["interval_ops_per_s"], "unit": "intervals/s", "desc": plain["desc"],
                        "ms_per_op": plain["ms_per_op"]},
         "batched_vs_plain": head["interval_ops_per_s"] / plain["interval_ops_per_s"],
+        "autotune": head.get("autotune"),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.batch),
                      "kernel": "setop fused union",

@@ -42,7 +42,7 @@ enum {
 };
 
 /* status word bits */
-enum { STX_FLAG_CAPACITY = 1, STX_FLAG_SPIN = 2, STX_FLAG_ALIGN = 4, STX_FLAG_ROWS = 8 };
+enum { STX_FLAG_CAPACITY = 1, STX_FLAG_SPIN = 2, STX_FLAG_ALIGN = 4, STX_FLAG_ROWS = 8, STX_FLAG_DOMAIN = 16 };
 
 /* operand transforms applied on load (set algebra / merges) */
 enum { STX_XF_NONE = 0, STX_XF_WIDEN = 1, STX_XF_COARSEN = 2, STX_XF_CLIP = 3, STX_XF_SCALE = 4 };
@@ -182,6 +182,18 @@ int32_t stx_stencil_seg_build(const int32_t* rows, const int32_t* off, int64_t R
 int32_t stx_upwind_seg(int32_t dtype, const int32_t* plan, int64_t ntiles, int64_t C, const int32_t* segs,
                        const void* u, void* out, double a, double b, double dt, double dx, double dy,
                        const void* inst_coef, hipStream_t stream);
+/* Gradient tags of an interval field through its segment table (the tag of
+ * the regrid step, regrid.py:40-120 / K13, as stx_grad_tag): runs of cells
+ * with |grad u| >= t, per row.  Every cell must lie in [0, w) of its row for
+ * the widths the table was built with (else STX_FLAG_DOMAIN).  words: 6 * W + 1
+ * uint32 of scratch, W = stx_grad_tag_seg_words(C); off / co / R: the set's
+ * row offsets, cell offsets and row count; outputs as stx_grad_tag. */
+int64_t stx_grad_tag_seg_words(int64_t cells);
+int32_t stx_grad_tag_seg(int32_t dtype, const int32_t* plan, int64_t ntiles, int64_t C, const int32_t* segs,
+                         const int32_t* off, const int32_t* co, int64_t R, const void* u, double t, uint32_t* words,
+                         int64_t W, int32_t* out_off, int32_t* out_beg, int32_t* out_end, int64_t capacity,
+                         int32_t* d_total, uint32_t* status, void* scratch, int64_t scratch_bytes, uint32_t epoch,
+                         hipStream_t stream);
 /* FP64 mass sum(u)*scale with a fixed reduction order (no reference
  * counterpart; conservation check of the AMR step).  part: stx_mass_parts(n)
  * doubles. */

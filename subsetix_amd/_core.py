@@ -12,7 +12,7 @@ from typing import Optional, Tuple
 
 import torch
 
-from . import _lib
+from . import _lib, autotune
 from ._lib import _I32, _I64, _P, check, context, ptr
 
 I32 = torch.int32
@@ -125,16 +125,25 @@ def setop_raw(op: int, a, b, *, nrows: int, d_nrows: Optional[torch.Tensor] = No
         ob, oe = buffers
     else:
         ob, oe = empty_i32(cap, dev), empty_i32(cap, dev)
-    g = group or _GROUP_OVERRIDE or 0
-    if g == 0 and (map_a is not None or map_b is not None or d_nrows is not None):
-        g = choose_group(na + nb, nrows)
-    nbytes = _lib.lib().stx_setop_scratch_bytes(nrows, g)
-    sp, sbytes, ep = ctx.scratch(nbytes)
     b_beg, b_end, b_off = b if b is not None else (None, None, None)
-    check(_lib.lib().stx_setop(int(op), ptr(a_beg), ptr(a_end), ptr(a_off), ptr(map_a), int(xf_a[0]), int(xf_a[1]),
-                               int(na), ptr(b_beg), ptr(b_end), ptr(b_off), ptr(map_b), int(xf_b[0]), int(xf_b[1]),
-                               int(nb), ptr(d_nrows), int(nrows), int(g), ptr(out_off), ptr(ob), ptr(oe), cap,
-                               ctx.slot(0), ctx.status_ptr, sp, sbytes, ep, ctx.stream), "setop")
+
+    def launch(g: int) -> None:
+        sp, sbytes, ep = ctx.scratch(_lib.lib().stx_setop_scratch_bytes(nrows, g))
+        check(_lib.lib().stx_setop(int(op), ptr(a_beg), ptr(a_end), ptr(a_off), ptr(map_a), int(xf_a[0]),
+                                   int(xf_a[1]), int(na), ptr(b_beg), ptr(b_end), ptr(b_off), ptr(map_b),
+                                   int(xf_b[0]), int(xf_b[1]), int(nb), ptr(d_nrows), int(nrows), int(g),
+                                   ptr(out_off), ptr(ob), ptr(oe), cap, ctx.slot(0), ctx.status_ptr, sp, sbytes, ep,
+                                   ctx.stream), "setop")
+
+    g = group or _GROUP_OVERRIDE or 0
+    if g == 0:
+        tile_ok = map_a is None and map_b is None and d_nrows is None
+        cands = (((-1, -2) if b is not None else (-1,)) if tile_ok else ()) + autotune.GROUPS
+        key = ("setop", int(op), tile_ok, autotune.bucket(na + nb), autotune.bucket((na + nb) / max(nrows, 1)))
+        g = autotune.pick(key, lambda: {c: (lambda c=c: launch(c)) for c in cands}) or 0
+        if g == 0 and not tile_ok:
+            g = choose_group(na + nb, nrows)
+    launch(int(g))
     return out_off, ob, oe, ctx
 
 

@@ -102,7 +102,7 @@ def ptr(t) -> int:
 # Per (device, stream) context: status word, look-back scratch, result slots.
 
 STATUS_BITS = {1: "output capacity exceeded", 2: "look-back wait bound exceeded", 4: "alignment violated",
-               8: "row ids out of range"}
+               8: "row ids out of range", 16: "cells outside the domain of the segment table"}
 
 
 class Context:

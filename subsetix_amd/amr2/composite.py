@@ -89,10 +89,13 @@ class CompositeTwoLevel:
         restrict_into(self.refine, self.coarse, self.fine_field, self.ratio, accumulate=self.accumulate)
         prolong_into(self.ghost, self.fine_field, self.coarse, self.ratio)
 
-    def prepare(self) -> None:
+    def prepare(self, tune: bool = False) -> None:
         """Build every cached table a step uses (cell offsets of the transfer
         targets, stencil segment tables), so that steps launch kernels only
-        and can be captured into a graph."""
+        and can be captured into a graph.  ``tune``: also run the step's
+        kernels once under the schedule autotuner (the state is left as it
+        was: the ghost prolongation is idempotent and the other kernels write
+        only the next buffers, which a step overwrites whole)."""
         from ..interval_field import cell_offsets
         from ..interval_stencil import stencil_plan
 
@@ -108,12 +111,31 @@ class CompositeTwoLevel:
                 transfer_plan(1, self.refine, self.coarse, self.fine_field)
         stencil_plan(self.coarse_set, self._c_co, self.width)
         stencil_plan(self.ext, self._f_co, self.width * self.ratio)
+        if tune:
+            from .. import autotune
 
-    def capture(self, stream: "torch.cuda.Stream") -> "torch.cuda.CUDAGraph":
+            with autotune.tuning():
+                self._kernels(self._cur)
+
+    def _kernels(self, cur: int) -> None:
+        """The kernels of one step from buffer ``cur`` into ``1 - cur``."""
+        a, b = self.velocity
+        c = IntervalField(self.coarse_set, self._cbuf[cur], self._c_co)
+        f = IntervalField(self.ext, self._fbuf[cur], self._f_co)
+        nc = IntervalField(self.coarse_set, self._cbuf[1 - cur], self._c_co)
+        nf = IntervalField(self.ext, self._fbuf[1 - cur], self._f_co)
+        prolong_into(self.ghost, f, c, self.ratio)
+        upwind_values(c, c.values, nc.values, width=self.width, height=self.height, a=a, b=b, dt=self.dt,
+                      dx=self.dx_c, dy=self.dy_c, check_rows=False)
+        upwind_values(f, f.values, nf.values, width=self.width * self.ratio, height=self.height * self.ratio, a=a,
+                      b=b, dt=self.dt, dx=self.dx_f, dy=self.dy_f, check_rows=False)
+        restrict_into(self.refine, nc, nf, self.ratio, accumulate=self.accumulate)
+
+    def capture(self, stream: "torch.cuda.Stream", tune: bool = False) -> "torch.cuda.CUDAGraph":
         """Graph of two steps from the current buffer (ends on it again)."""
         from .._lib import context
 
-        self.prepare()
+        self.prepare(tune)
         stream.wait_stream(torch.cuda.current_stream(stream.device))
         with torch.cuda.stream(stream):
             context(stream.device)  # per-stream context allocated outside the capture
@@ -127,16 +149,8 @@ class CompositeTwoLevel:
         return g
 
     def step(self) -> None:
-        a, b = self.velocity
-        c, f = self.coarse, self.fine_field
-        prolong_into(self.ghost, f, c, self.ratio)
-        nc, nf = self._cbuf[1 - self._cur], self._fbuf[1 - self._cur]
-        upwind_values(c, c.values, nc, width=self.width, height=self.height, a=a, b=b, dt=self.dt, dx=self.dx_c,
-                      dy=self.dy_c, check_rows=False)
-        upwind_values(f, f.values, nf, width=self.width * self.ratio, height=self.height * self.ratio, a=a, b=b,
-                      dt=self.dt, dx=self.dx_f, dy=self.dy_f, check_rows=False)
+        self._kernels(self._cur)
         self._cur = 1 - self._cur
-        restrict_into(self.refine, self.coarse, self.fine_field, self.ratio, accumulate=self.accumulate)
 
     def mass(self) -> float:
         """FP64 composite mass (coarse level after the restriction)."""

@@ -19,10 +19,18 @@ gradient tag on L0, grading dilation, one-cell L1 ring forced onto its
 parents, gradient tag on L1 gated by the eroded L1 (3-level nesting:
 L2 inside L1, no 8-neighbour contact between L2 and cells outside L1,
 coarse parents of L2 refined at L1).
+
+With ``BatchConfig.autotune`` every kernel call of the first regrid(s) and a
+tuning pass over the step kernels run under the schedule autotuner
+(``subsetix_amd.autotune``): each shape bucket times its candidate schedules
+(segment tiles or row groups of 4 / 16 / 64 lanes, fused or two-pass set
+kernels) and the captured steps use the fastest; later regrids take the
+nearest tuned bucket's choice.
 """
 
 from __future__ import annotations
 
+import contextlib
 import math
 import time
 from dataclasses import dataclass, field
@@ -31,7 +39,7 @@ from typing import Dict, List, Optional
 import numpy as np
 import torch
 
-from . import _core, _lib
+from . import _core, _lib, autotune
 from ._lib import _I32, _I64, _P, check, context, ptr
 from .expressions import IntervalSet, binary_op
 from .interval_field import IntervalField, cell_offsets, create_interval_field
@@ -61,6 +69,8 @@ class BatchConfig:
     regrid_every: int = 8
     use_graph: bool = True
     accumulate: str = "f64"
+    autotune: bool = False  # time the candidate schedules of every kernel per shape bucket (regrid + step)
+    autotune_regrids: int = 1  # regrids that tune; later ones reuse the nearest tuned buckets
 
 
 class BatchedAMR:
@@ -110,6 +120,8 @@ class BatchedAMR:
         self.graph = None
         self.step_count = 0
         self.timings: Dict[str, float] = {}
+        self._regrids = 0
+        self._plan0 = None
         self.regrid()
         self._last_regrid = 0
 
@@ -141,14 +153,28 @@ class BatchedAMR:
         return out
 
     # ------------------------------------------------------------------ regrid
-    def _tag(self, fld: IntervalField, level: int) -> IntervalSet:
+    def _tag(self, fld: IntervalField, level: int, plan=None) -> IntervalSet:
         # per-cell threshold scaled to the level's resolution: |grad u| per
         # cell halves when the cell size halves
         t = self.cfg.threshold[min(level, len(self.cfg.threshold) - 1)]
-        return gradient_tag(fld, t)
+        return gradient_tag(fld, t, plan=plan)
 
     def regrid(self) -> None:
         t0 = time.perf_counter()
+        tune = self.cfg.autotune and self._regrids < self.cfg.autotune_regrids
+        self._regrids += 1
+        with autotune.tuning() if tune else contextlib.nullcontext():
+            self._regrid()
+            if tune:
+                # tuning pass over the step kernels on the new geometry, before
+                # any capture: it leaves the state as it was (the ghost
+                # prolongation is idempotent, the other kernels write only the
+                # next buffers, which a step overwrites whole)
+                self._step_kernels(self.cur)
+        torch.cuda.synchronize(self.dev)
+        self.timings["regrid_s"] = self.timings.get("regrid_s", 0.0) + time.perf_counter() - t0
+
+    def _regrid(self) -> None:
         R = self.cfg.ratio
         p = self.cfg.grading
         L = self.cfg.levels
@@ -156,7 +182,12 @@ class BatchedAMR:
         cur0 = IntervalField(D[0], self.buf[0][self.cur], self.f[0].interval_cell_offsets)
         old = self._fields(self.cur) if hasattr(self, "ext") else list(self.f)
         old_sets = list(getattr(self, "level_sets", []))
-        tag0 = self._tag(cur0, 0)
+        if self._plan0 is None:
+            # level 0 is the fixed full domain of every instance: one segment
+            # table serves its gradient tags and its stencil for the whole run
+            self._plan0 = build_stencil_plan(D[0], self.f[0].interval_cell_offsets, 0, self._row_inst(D[0], 0),
+                                             self.inst_w[0])
+        tag0 = self._tag(cur0, 0, self._plan0)
         refine0 = binary_op(dilate_interval_set(tag0, p, p) if p > 0 else tag0, D[0], "intersection")
         sets = [D[0]]
         refines = []
@@ -204,12 +235,10 @@ class BatchedAMR:
                     transfer_plan(0, self.ghosts[lv], fl[lv], fl[lv - 1])
                 if self.refines[lv - 1].interval_count:
                     transfer_plan(1, self.refines[lv - 1], fl[lv - 1], fl[lv])
-        self.plans = [build_stencil_plan(self.ext[lv], self.f[lv].interval_cell_offsets, 0, self.row_inst[lv],
-                                         self.inst_w[lv]) for lv in range(L)]
+        self.plans = [self._plan0] + [build_stencil_plan(self.ext[lv], self.f[lv].interval_cell_offsets, 0,
+                                                         self.row_inst[lv], self.inst_w[lv]) for lv in range(1, L)]
         self.graph = None
         self.cells = self.cells_per_step()
-        torch.cuda.synchronize(self.dev)
-        self.timings["regrid_s"] = self.timings.get("regrid_s", 0.0) + time.perf_counter() - t0
 
     def _level_field(self, lv, target, coarser: IntervalField, old, old_sets, ghost) -> IntervalField:
         """Field on `target` at level lv: prolonged from the coarser level,

@@ -14,7 +14,8 @@
 // Segments are also cut at the domain edges x = 0 and x = w (the row's width:
 // the call's width, or the instance width for a stack of instances), so the
 // zero-exterior range checks fold into per-segment flags.  Segment record (8
-// int32): c0, c_end, dd, ud, flags, instance, 0, 0.  flags bit 0: c0 has a
+// int32): c0, c_end, dd, ud, flags, instance, x0 (column of c0), row
+// ordinal.  flags bit 0: c0 has a
 // left neighbour (cell c0 - 1); bit 1: the segment lies in [0, w), so inner
 // cells have both horizontal neighbours; bit 2: c_end - 1 has a right
 // neighbour (cell c_end).  dd / ud = kNone when there is no vertical
@@ -122,8 +123,8 @@ __global__ void segments_kernel(const int32_t* __restrict__ rows, const int32_t*
             rec[3] = (inside && ju < jue && beg[ju] <= x) ? (int32_t)(((int64_t)co[ju] - beg[ju]) - cself) : kNone;
             rec[4] = (lf ? 1 : 0) | (inside ? 2 : 0) | (rl ? 4 : 0);
             rec[5] = inst;
-            rec[6] = 0;
-            rec[7] = 0;
+            rec[6] = x;
+            rec[7] = (int32_t)k;
         }
         ++n;
         x = nx;
@@ -337,6 +338,224 @@ __global__ __launch_bounds__(kThreads) void upwind_seg_kernel(const int32_t* __r
     }
 }
 
+// ---------------------------------------------------------------------------
+// Gradient tags over a segment table (the regrid tag of a level whose cells
+// all lie in [0, w) of their row: every segment is then "inside" and its
+// flags and vertical deltas are exactly the in-set neighbours that the
+// row-group kernel GradTagRow finds by walking the rows).  Three streaming
+// passes over bit masks, 32 cells per word:
+//   1. tag_bits: per cell, tag = |grad u| >= t (central differences where
+//      both neighbours exist, one-sided otherwise) and lex = the cell has a
+//      left neighbour; two words per 32 cells;
+//   2. tag_runs: run starts S = tag & ~(left tag & lex), run ends
+//      E = tag & ~(right tag & right exists), popcount of S per word (then an
+//      exclusive scan of the counts);
+//   3. tag_write: the r-th start / end of the field in storage order is the
+//      r-th run's begin / end (x from the cell's segment: x0 + c - c0), and
+//      the row offsets are the start counts before each row's first cell.
+template <typename T>
+__device__ __forceinline__ bool grad_tag_cell(T center, T left, T right, T down, T up, bool hl, bool hr, bool hd,
+                                              bool hu, T t) {
+    T gx = (T)0, gy = (T)0;
+    if (hl && hr) gx = (right - left) * (T)0.5;
+    else if (hr) gx = right - center;
+    else if (hl) gx = center - left;
+    if (hd && hu) gy = (up - down) * (T)0.5;
+    else if (hu) gy = up - center;
+    else if (hd) gy = center - down;
+    const T gx2 = gx * gx;
+    const T gy2 = gy * gy;
+    return sqrt(gx2 + gy2) >= t;
+}
+
+// Tag and left-neighbour bits of the group [c, c + G) (bit v = cell c + v).
+template <typename T, class SG>
+__device__ __forceinline__ void tag_group(const SG& sg, int32_t s0, int32_t s1, int32_t c, int32_t C,
+                                          __amdgpu_buffer_rsrc_t rs, const T* __restrict__ u, T thr,
+                                          uint32_t& tags, uint32_t& lex, bool& outside) {
+    using VT = typename Vec<T>::type;
+    constexpr int G = Vec<T>::n;
+    int32_t lo = s0, hi = s1;
+    while (hi - lo > 1) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (sg.c0(mid) <= c) lo = mid; else hi = mid;
+    }
+    int32_t s = lo;
+    Seg g = sg.get(s);
+    tags = 0u;
+    lex = 0u;
+    if (c + G <= C && c + G <= g.cend) {
+        const VT cv = *(const VT*)(u + c);
+        const VT dv = load_dword_aligned(u + (g.dd != kNone ? c + g.dd : c));
+        const VT uv = load_dword_aligned(u + (g.ud != kNone ? c + g.ud : c));
+        const T lv = gather<T>(rs, c - 1);
+        const T rv = gather<T>(rs, c + G);
+        const bool inner = g.flags & 2;
+        outside |= !inner;
+        const bool lfirst = c == g.c0 ? (g.flags & 1) : inner;
+        const bool rlast = c + G == g.cend ? (g.flags & 4) : inner;
+        const bool hd = g.dd != kNone, hu = g.ud != kNone;
+#pragma unroll
+        for (int v = 0; v < G; ++v) {
+            const bool lok = v == 0 ? lfirst : inner;
+            const bool rok = v == G - 1 ? rlast : inner;
+            const T left = v == 0 ? lv : cv[v - 1];
+            const T right = v == G - 1 ? rv : cv[v + 1];
+            if (grad_tag_cell<T>(cv[v], left, right, dv[v], uv[v], lok, rok, hd, hu, thr)) tags |= 1u << v;
+            if (lok) lex |= 1u << v;
+        }
+        return;
+    }
+#pragma unroll
+    for (int v = 0; v < G; ++v) {
+        const int32_t cc = c + v;
+        if (cc < C) {
+            if (cc >= g.cend) {
+                ++s;
+                g = sg.get(s);
+            }
+            outside |= !(g.flags & 2);
+            const bool lok = (cc == g.c0) ? (g.flags & 1) : (g.flags & 2);
+            const bool rok = (cc + 1 == g.cend) ? (g.flags & 4) : (g.flags & 2);
+            const T center = u[cc];
+            const T left = gather<T>(rs, cc - 1);
+            const T right = gather<T>(rs, cc + 1);
+            const T down = gather<T>(rs, g.dd != kNone ? cc + g.dd : -1);
+            const T up = gather<T>(rs, g.ud != kNone ? cc + g.ud : -1);
+            if (grad_tag_cell<T>(center, left, right, down, up, lok, rok, g.dd != kNone, g.ud != kNone, thr))
+                tags |= 1u << v;
+            if (lok) lex |= 1u << v;
+        }
+    }
+}
+
+// Pass 1.  Lanes own consecutive G-cell groups, so 32 / G consecutive lanes
+// hold one 32-cell word: their bits are OR-reduced across the lanes and the
+// first lane of each word writes it.  Every lane of a block takes part in the
+// reductions (lanes past the last cell contribute zero bits).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void tag_bits_kernel(const int32_t* __restrict__ plan, int64_t ntiles,
+                                                            int32_t C, const int32_t* __restrict__ segs,
+                                                            const T* __restrict__ u, T thr,
+                                                            uint32_t* __restrict__ tag_w, uint32_t* __restrict__ lex_w,
+                                                            int64_t W, uint32_t* __restrict__ status) {
+    constexpr int G = Vec<T>::n;
+    constexpr int LPW = 32 / G;  // lanes per word
+    __shared__ int4 s_seg[2 * kSegCap];
+    const int64_t per = (int64_t)gridDim.x / kXcds;
+    const int64_t t = (int64_t)(blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+    if (t >= ntiles) return;
+    const int32_t s0 = plan[2 * t], s1 = plan[2 * t + 1];
+    const int32_t M = s1 - s0;
+    const int32_t ct = (int32_t)(t * kTileCells);
+    const int4* gs = (const int4*)segs;
+    const bool lds = M <= kSegCap;
+    if (lds) {
+        for (int q = threadIdx.x; q < 2 * M; q += kThreads) s_seg[q] = gs[2 * (int64_t)s0 + q];
+        __syncthreads();
+    }
+    const __amdgpu_buffer_rsrc_t rs = field_rsrc(u, (int64_t)C * (int64_t)sizeof(T));
+    const int lane = lane_id();
+    bool outside = false;
+#pragma unroll
+    for (int q = 0; q < kCellsPerThread / G; ++q) {
+        const int32_t c = ct + (q * kThreads + (int32_t)threadIdx.x) * G;
+        uint32_t tg = 0u, lx = 0u;
+        if (c < C) {
+            if (lds)
+                tag_group<T>(LdsSegs{s_seg, s0}, s0, s1, c, C, rs, u, thr, tg, lx, outside);
+            else
+                tag_group<T>(GlobalSegs{gs}, s0, s1, c, C, rs, u, thr, tg, lx, outside);
+        }
+        const int sh = (lane % LPW) * G;
+        uint32_t tw = tg << sh, lw = lx << sh;
+#pragma unroll
+        for (int d = 1; d < LPW; d <<= 1) {
+            tw |= (uint32_t)__shfl_xor((int)tw, d, 64);
+            lw |= (uint32_t)__shfl_xor((int)lw, d, 64);
+        }
+        const int64_t w = (int64_t)c >> 5;
+        if (lane % LPW == 0 && w < W) {
+            tag_w[w] = tw;
+            lex_w[w] = lw;
+        }
+    }
+    if (outside) atomicOr(status, STX_FLAG_DOMAIN);
+}
+
+// Pass 2: run start / end words and the start count of every word.
+__global__ void tag_runs_kernel(const uint32_t* __restrict__ tag_w, const uint32_t* __restrict__ lex_w, int64_t W,
+                                uint32_t* __restrict__ st_w, uint32_t* __restrict__ en_w, int32_t* __restrict__ cnt) {
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= W) return;
+    const uint32_t T = tag_w[w], L = lex_w[w];
+    const uint32_t tp = w > 0 ? tag_w[w - 1] >> 31 : 0u;
+    const uint32_t tn = w + 1 < W ? tag_w[w + 1] & 1u : 0u;
+    const uint32_t ln = w + 1 < W ? lex_w[w + 1] & 1u : 0u;
+    const uint32_t S = T & ~(((T << 1) | tp) & L);
+    const uint32_t rx = (L >> 1) | (ln << 31);  // cell has a right neighbour
+    const uint32_t E = T & ~(((T >> 1) | (tn << 31)) & rx);
+    st_w[w] = S;
+    en_w[w] = E;
+    cnt[w] = __popc(S);
+}
+
+// Column of cell c from its segment (plan tile, then a binary search).
+__device__ __forceinline__ int32_t cell_x(const int32_t* __restrict__ plan, const int32_t* __restrict__ segs,
+                                          int32_t c) {
+    const int64_t t = c / kTileCells;
+    int32_t lo = plan[2 * t], hi = plan[2 * t + 1];
+    while (hi - lo > 1) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (segs[kRec * (int64_t)mid] <= c) lo = mid; else hi = mid;
+    }
+    return segs[kRec * (int64_t)lo + 6] + (c - segs[kRec * (int64_t)lo]);
+}
+
+// Pass 3a: begins / ends of the runs.  pref: exclusive scan of the start
+// counts; a run still open at the first cell of word w (its last cell in an
+// earlier word) shifts the end ranks of the word by one.
+__global__ void tag_write_kernel(const uint32_t* __restrict__ tag_w, const uint32_t* __restrict__ st_w,
+                                 const uint32_t* __restrict__ en_w, const int32_t* __restrict__ pref, int64_t W,
+                                 const int32_t* __restrict__ plan, const int32_t* __restrict__ segs,
+                                 int32_t* __restrict__ ob, int32_t* __restrict__ oe, int64_t capacity,
+                                 uint32_t* __restrict__ status) {
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= W) return;
+    uint32_t S = st_w[w], E = en_w[w];
+    if ((S | E) == 0u) return;
+    const int64_t base = pref[w];
+    const int64_t open = (w > 0 && (tag_w[w - 1] >> 31) && !(en_w[w - 1] >> 31)) ? 1 : 0;
+    const int32_t c0 = (int32_t)(w << 5);
+    for (int64_t r = base; S; ++r) {
+        const int b = __ffs((int)S) - 1;
+        S &= S - 1u;
+        if (r < capacity) ob[r] = cell_x(plan, segs, c0 + b);
+        else atomicOr(status, STX_FLAG_CAPACITY);
+    }
+    for (int64_t r = base - open; E; ++r) {
+        const int b = __ffs((int)E) - 1;
+        E &= E - 1u;
+        if (r < capacity) oe[r] = cell_x(plan, segs, c0 + b) + 1;
+        else atomicOr(status, STX_FLAG_CAPACITY);
+    }
+}
+
+__global__ void zero_total_kernel(int32_t* __restrict__ p) { *p = 0; }
+
+// Pass 3b: out_off[k] = starts before the first cell of row k (k = R: all).
+__global__ void tag_rows_kernel(const int32_t* __restrict__ off, const int32_t* __restrict__ co, int64_t R,
+                                int32_t C, const uint32_t* __restrict__ st_w, const int32_t* __restrict__ pref,
+                                const int32_t* __restrict__ cnt, int64_t W, int32_t* __restrict__ out_off) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > R) return;
+    const int32_t total = W > 0 ? pref[W - 1] + cnt[W - 1] : 0;
+    const int32_t c = k == R ? C : co[off[k]];
+    const int64_t w = (int64_t)c >> 5;
+    const int b = c & 31;
+    out_off[k] = w >= W ? total : pref[w] + __popc(st_w[w] & ((1u << b) - 1u));
+}
+
 template <typename T, bool BATCH>
 int32_t launch_upwind_seg(const int32_t* plan, int64_t ntiles, int64_t C, const int32_t* segs, const void* u,
                           void* out, double a, double b, double dt, double dx, double dy, const void* inst_coef,
@@ -417,6 +636,52 @@ int32_t stx_upwind_seg(int32_t dtype, const int32_t* plan, int64_t ntiles, int64
                      : launch_upwind_seg<double, false>(plan, ntiles, C, segs, u, out, a, b, dt, dx, dy,
                                                         inst_coef, stream);
     return stx_fail(STX_EINVAL, "upwind_seg: dtype must be 4 or 8");
+}
+
+int64_t stx_grad_tag_seg_words(int64_t cells) { return (cells + 31) / 32; }
+
+int32_t stx_grad_tag_seg(int32_t dtype, const int32_t* plan, int64_t ntiles, int64_t C, const int32_t* segs,
+                         const int32_t* off, const int32_t* co, int64_t R, const void* u, double t, uint32_t* words,
+                         int64_t W, int32_t* out_off, int32_t* out_beg, int32_t* out_end, int64_t capacity,
+                         int32_t* d_total, uint32_t* status, void* scratch, int64_t scratch_bytes, uint32_t epoch,
+                         hipStream_t stream) {
+    if (C < 0 || R < 0 || ntiles < 0) return stx_fail(STX_EINVAL, "grad_tag_seg: bad sizes");
+    if (W != (C + 31) / 32) return stx_fail(STX_EINVAL, "grad_tag_seg: word count must be ceil(cells / 32)");
+    if (ntiles != (C + kTileCells - 1) / kTileCells) return stx_fail(STX_EINVAL, "grad_tag_seg: plan size mismatch");
+    if (C > 0x7fffffffLL - 2 * kTileCells) return stx_fail(STX_EINVAL, "grad_tag_seg: too many cells");
+    if (C * (dtype == 8 ? 8 : 4) > 0xffffff00LL)
+        return stx_fail(STX_EINVAL, "grad_tag_seg: field larger than 4 GiB (use the row-group kernel)");
+    if ((uintptr_t)u % 16 != 0) return stx_fail(STX_EINVAL, "grad_tag_seg: values must be 16-byte aligned");
+    // words (6 W + 1): tag, lex, start, end, count (W each), prefix (W + 1)
+    uint32_t* tag_w = words;
+    uint32_t* lex_w = words + W;
+    uint32_t* st_w = words + 2 * W;
+    uint32_t* en_w = words + 3 * W;
+    int32_t* cnt = (int32_t*)(words + 4 * W);
+    int32_t* pref = (int32_t*)(words + 5 * W);
+    if (W > 0) {
+        const int64_t grid = (ntiles + kXcds - 1) / kXcds * kXcds;
+        if (dtype == 4)
+            hipLaunchKernelGGL(tag_bits_kernel<float>, dim3((unsigned)grid), dim3(kThreads), 0, stream, plan, ntiles,
+                               (int32_t)C, segs, (const float*)u, (float)t, tag_w, lex_w, W, status);
+        else if (dtype == 8)
+            hipLaunchKernelGGL(tag_bits_kernel<double>, dim3((unsigned)grid), dim3(kThreads), 0, stream, plan, ntiles,
+                               (int32_t)C, segs, (const double*)u, t, tag_w, lex_w, W, status);
+        else
+            return stx_fail(STX_EINVAL, "grad_tag_seg: dtype must be 4 or 8");
+        hipLaunchKernelGGL(tag_runs_kernel, dim3(nblocks(W)), dim3(256), 0, stream, tag_w, lex_w, W, st_w, en_w, cnt);
+        int32_t rc = stx_launch_status("grad_tag_seg");
+        if (rc) return rc;
+        rc = stx_scan_i32(cnt, nullptr, W, pref, d_total, status, scratch, scratch_bytes, epoch, stream);
+        if (rc) return rc;
+        hipLaunchKernelGGL(tag_write_kernel, dim3(nblocks(W)), dim3(256), 0, stream, tag_w, st_w, en_w, pref, W, plan,
+                           segs, out_beg, out_end, capacity, status);
+    } else {
+        hipLaunchKernelGGL(zero_total_kernel, dim3(1), dim3(1), 0, stream, d_total);
+    }
+    hipLaunchKernelGGL(tag_rows_kernel, dim3(nblocks(R + 1)), dim3(256), 0, stream, off, co, R, (int32_t)C, st_w, pref,
+                       cnt, W, out_off);
+    return stx_launch_status("grad_tag_seg");
 }
 
 }  // extern "C"

@@ -28,6 +28,7 @@ enum : uint32_t {
     STX_FLAG_SPIN = 2u,
     STX_FLAG_ALIGN = 4u,
     STX_FLAG_ROWS = 8u,
+    STX_FLAG_DOMAIN = 16u,  // cells outside the [0, w) domain a segment table was built for
 };
 
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (STX_WAVE - 1)); }

@@ -11,7 +11,7 @@ from __future__ import annotations
 
 import torch
 
-from . import _core, _lib
+from . import _core, _lib, autotune
 from ._lib import _F64, _I32, _I64, _P, check, context, ptr
 from .expressions import IntervalSet
 from .interval_field import IntervalField, cell_offsets
@@ -28,6 +28,9 @@ _lib.register("stx_transfer_seg_count", _I32, [_I32] + _GEOM7 + [_P, _I64] + _GE
 _lib.register("stx_transfer_seg_build", _I32, [_I32] + _GEOM7 + [_P, _I64] + _GEOM7 + _GEOM7
               + [_P, _I64, _P, _I64, _P, _I64, _P])
 _lib.register("stx_transfer_seg", _I32, [_I32, _I32, _I32, _P, _I64, _I64, _P, _P, _I64, _P, _P])
+_lib.register("stx_grad_tag_seg_words", _I64, [_I64])
+_lib.register("stx_grad_tag_seg", _I32, [_I32, _P, _I64, _I64, _P, _P, _P, _I64, _P, _F64, _P, _I64, _P, _P, _P, _I64,
+                                         _P, _P, _P, _I64, _I32, _P])
 
 
 def _geom(s: IntervalSet, co: torch.Tensor):
@@ -83,21 +86,58 @@ def transfer_plan(mode: int, target: IntervalSet, dst: IntervalField, src: Inter
     return out
 
 
-_TRANSFER_SCHEDULE = "segments"
+_TRANSFER_SCHEDULE = "auto"
 
 
 def set_transfer_schedule(name: str) -> None:
-    """'segments' (segment tables, ratio 2; default) or 'rows' (lane group
-    per target row, any ratio)."""
+    """'auto' (default: the autotuner's choice when it has one, else segment
+    tables at ratio 2), 'segments' (segment tables, ratio 2) or 'rows' (lane
+    group per target row, any ratio)."""
     global _TRANSFER_SCHEDULE
-    if name not in ("segments", "rows"):
-        raise ValueError("schedule must be 'segments' or 'rows'")
+    if name not in ("auto", "segments", "rows"):
+        raise ValueError("schedule must be 'auto', 'segments' or 'rows'")
     _TRANSFER_SCHEDULE = name
 
 
-def _seg_ok(dst: IntervalField, src: IntervalField, ratio: int) -> bool:
-    return (_TRANSFER_SCHEDULE == "segments" and ratio == 2
-            and src.values.numel() * src.values.element_size() < (1 << 32) - 256)
+def _seg_possible(src: IntervalField, ratio: int) -> bool:
+    return ratio == 2 and src.values.numel() * src.values.element_size() < (1 << 32) - 256
+
+
+def _transfer(mode: int, target: IntervalSet, dst: IntervalField, src: IntervalField, ratio: int, acc64: int) -> None:
+    """One fused transfer with the schedule chosen as for the stencil:
+    forced, else the autotuner's (candidates run on a copy of the
+    destination), else segment tables when they apply."""
+    ctx = context(dst.values.device)
+    dcode = _dtype_code(dst.values)
+    seg_ok = _seg_possible(src, ratio)
+    group_cells = int(dst.values.numel()) if mode == 0 else int(src.values.numel())
+
+    def launch(choice, d: IntervalField) -> None:
+        if choice == "seg":
+            plan, ntiles, cells, segs = transfer_plan(mode, target, d, src)
+            check(_lib.lib().stx_transfer_seg(mode, dcode, acc64, ptr(plan), ntiles, cells, ptr(segs),
+                                              ptr(src.values), int(src.values.numel()), ptr(d.values), ctx.stream),
+                  "prolong_into" if mode == 0 else "restrict_into")
+            return
+        tco = cell_offsets(target)
+        check(_lib.lib().stx_transfer(mode, dcode, acc64, int(choice), int(ratio), *_geom(target, tco),
+                                      *_geom(d.interval_set, d.interval_cell_offsets),
+                                      *_geom(src.interval_set, src.interval_cell_offsets), ptr(src.values),
+                                      ptr(d.values), ctx.stream), "prolong_into" if mode == 0 else "restrict_into")
+
+    fixed = "seg" if (seg_ok and _TRANSFER_SCHEDULE != "rows") else _group(group_cells, target.row_count)
+    choice = fixed
+    if _TRANSFER_SCHEDULE == "auto":
+        def factory():
+            scratch = IntervalField(dst.interval_set, dst.values.clone(), dst.interval_cell_offsets)
+            cands = (("seg",) if seg_ok else ()) + autotune.GROUPS
+            return {c: (lambda c=c: launch(c, scratch)) for c in cands}
+
+        key = ("transfer", mode, dcode, acc64, int(ratio), seg_ok, autotune.bucket(target.interval_count),
+               autotune.bucket(group_cells))
+        tuned = autotune.pick(key, factory)
+        choice = fixed if tuned is None else tuned
+    launch(choice, dst)
 
 
 def prolong_into(target: IntervalSet, dst: IntervalField, src: IntervalField, ratio: int) -> None:
@@ -107,19 +147,7 @@ def prolong_into(target: IntervalSet, dst: IntervalField, src: IntervalField, ra
         return
     if dst.values.dtype != src.values.dtype:
         raise TypeError("source and destination fields must share a dtype")
-    ctx = context(dst.values.device)
-    if _seg_ok(dst, src, ratio):
-        plan, ntiles, cells, segs = transfer_plan(0, target, dst, src)
-        check(_lib.lib().stx_transfer_seg(0, _dtype_code(dst.values), 0, ptr(plan), ntiles, cells, ptr(segs),
-                                          ptr(src.values), int(src.values.numel()), ptr(dst.values), ctx.stream),
-              "prolong_into")
-        return
-    tco = cell_offsets(target)
-    n = int(dst.values.numel())
-    check(_lib.lib().stx_transfer(0, _dtype_code(dst.values), 0, _group(n, target.row_count), int(ratio),
-                                  *_geom(target, tco), *_geom(dst.interval_set, dst.interval_cell_offsets),
-                                  *_geom(src.interval_set, src.interval_cell_offsets), ptr(src.values),
-                                  ptr(dst.values), ctx.stream), "prolong_into")
+    _transfer(0, target, dst, src, int(ratio), 0)
 
 
 def restrict_into(target: IntervalSet, dst: IntervalField, src: IntervalField, ratio: int, *,
@@ -130,24 +158,17 @@ def restrict_into(target: IntervalSet, dst: IntervalField, src: IntervalField, r
         return
     if dst.values.dtype != src.values.dtype:
         raise TypeError("source and destination fields must share a dtype")
-    ctx = context(dst.values.device)
-    acc64 = 1 if accumulate == "f64" else 0
-    if _seg_ok(dst, src, ratio):
-        plan, ntiles, cells, segs = transfer_plan(1, target, dst, src)
-        check(_lib.lib().stx_transfer_seg(1, _dtype_code(dst.values), acc64, ptr(plan), ntiles, cells, ptr(segs),
-                                          ptr(src.values), int(src.values.numel()), ptr(dst.values), ctx.stream),
-              "restrict_into")
-        return
-    tco = cell_offsets(target)
-    check(_lib.lib().stx_transfer(1, _dtype_code(dst.values), acc64, _group(int(src.values.numel()), target.row_count),
-                                  int(ratio), *_geom(target, tco), *_geom(dst.interval_set, dst.interval_cell_offsets),
-                                  *_geom(src.interval_set, src.interval_cell_offsets), ptr(src.values),
-                                  ptr(dst.values), ctx.stream), "restrict_into")
+    _transfer(1, target, dst, src, int(ratio), 1 if accumulate == "f64" else 0)
 
 
-def gradient_tag(field: IntervalField, threshold: float) -> IntervalSet:
+def gradient_tag(field: IntervalField, threshold: float, plan=None) -> IntervalSet:
     """Runs of cells with |grad u| >= threshold (central differences where
-    both neighbours are in the set, one-sided otherwise), rows of the field."""
+    both neighbours are in the set, one-sided otherwise), rows of the field.
+    ``plan``: the field's stencil segment table (interval_stencil.
+    build_stencil_plan) for widths that contain every cell of the set; it
+    enables the segment schedule (bit masks, no row walks)."""
+    from .interval_stencil import _segment_kernel_ok
+
     s = field.interval_set
     dev = field.values.device
     ctx = context(dev)
@@ -156,11 +177,30 @@ def gradient_tag(field: IntervalField, threshold: float) -> IntervalSet:
     out_off = _core.empty_i32(s.row_count + 1, dev)
     ob = _core.empty_i32(cap, dev)
     oe = _core.empty_i32(cap, dev)
-    g = _group(n, s.row_count)
-    sp, sbytes, ep = ctx.scratch(_lib.lib().stx_setop_scratch_bytes(s.row_count, 4))
-    check(_lib.lib().stx_grad_tag(_dtype_code(field.values), g, ptr(s.rows), ptr(s.row_offsets), s.row_count,
-                                  ptr(s.begin), ptr(s.end), ptr(field.interval_cell_offsets), ptr(field.values), float(threshold), ptr(out_off), ptr(ob), ptr(oe), cap, ctx.slot(0),
-                                  ctx.status_ptr, sp, sbytes, ep, ctx.stream), "grad_tag")
+    dcode = _dtype_code(field.values)
+    seg_ok = plan is not None and plan[2] == n and _segment_kernel_ok(field.values, field.values)
+
+    def launch(g) -> None:
+        if g == "seg":
+            p, ntiles, ncells, segs = plan
+            nw = int(_lib.lib().stx_grad_tag_seg_words(ncells))
+            words = _core.empty_i32(6 * nw + 1, dev)  # tag, lex, start, end, count, prefix (+ total)
+            sp, sbytes, ep = ctx.scratch(_lib.lib().stx_scan_scratch_bytes(nw))
+            check(_lib.lib().stx_grad_tag_seg(dcode, ptr(p), ntiles, ncells, ptr(segs), ptr(s.row_offsets),
+                                              ptr(field.interval_cell_offsets), s.row_count, ptr(field.values),
+                                              float(threshold), ptr(words), nw, ptr(out_off), ptr(ob), ptr(oe), cap,
+                                              ctx.slot(0), ctx.status_ptr, sp, sbytes, ep, ctx.stream), "grad_tag")
+            return
+        sp, sbytes, ep = ctx.scratch(_lib.lib().stx_setop_scratch_bytes(s.row_count, g))
+        check(_lib.lib().stx_grad_tag(dcode, int(g), ptr(s.rows), ptr(s.row_offsets), s.row_count, ptr(s.begin),
+                                      ptr(s.end), ptr(field.interval_cell_offsets), ptr(field.values),
+                                      float(threshold), ptr(out_off), ptr(ob), ptr(oe), cap, ctx.slot(0),
+                                      ctx.status_ptr, sp, sbytes, ep, ctx.stream), "grad_tag")
+
+    cands = (("seg",) if seg_ok else ()) + autotune.GROUPS
+    key = ("grad_tag", dcode, seg_ok, autotune.bucket(n), autotune.bucket(n / max(s.row_count, 1)))
+    g = autotune.pick(key, lambda: {c: (lambda c=c: launch(c)) for c in cands})
+    launch(("seg" if seg_ok else _group(n, s.row_count)) if g is None else g)
     (total,) = _core.read_counts(ctx, ctx.words[1])
     return IntervalSet._trusted(ob[:total], oe[:total], out_off, s.rows, dense=s.is_dense_rows())
 
@@ -170,17 +210,33 @@ def upwind_batched(field: IntervalField, values: torch.Tensor, out: torch.Tensor
     """One upwind step over a stack of instances (see stx_upwind_batched);
     ``plan`` from interval_stencil.build_stencil_plan(set, co, row_inst,
     inst_w) selects the segment kernel."""
-    from .interval_stencil import _segment_kernel_ok, stencil_schedule
+    from . import interval_stencil
+    from .interval_stencil import _segment_kernel_ok
 
+    sched = interval_stencil.stencil_schedule()
     s = field.interval_set
     ctx = context(values.device)
-    if plan is not None and stencil_schedule() == "tiled" and _segment_kernel_ok(values, out):
-        p, ntiles, cells, segs = plan
-        check(_lib.lib().stx_upwind_seg(_dtype_code(values), ptr(p), ntiles, cells, ptr(segs), ptr(values), ptr(out),
-                                        0.0, 0.0, 0.0, 1.0, 1.0, ptr(inst_coef), ctx.stream), "upwind_batched")
-        return out
-    check(_lib.lib().stx_upwind_batched(_dtype_code(values), _group(int(values.numel()), s.row_count), ptr(s.rows),
-                                        ptr(s.row_offsets), s.row_count, ptr(s.begin), ptr(s.end),
-                                        ptr(field.interval_cell_offsets), ptr(values), ptr(out), ptr(row_inst),
-                                        ptr(inst_w), ptr(inst_coef), ctx.stream), "upwind_batched")
+    dcode = _dtype_code(values)
+    cells = int(values.numel())
+
+    def launch(choice) -> None:
+        if choice == "seg":
+            p, ntiles, ncells, segs = plan
+            check(_lib.lib().stx_upwind_seg(dcode, ptr(p), ntiles, ncells, ptr(segs), ptr(values), ptr(out), 0.0, 0.0,
+                                            0.0, 1.0, 1.0, ptr(inst_coef), ctx.stream), "upwind_batched")
+        else:
+            check(_lib.lib().stx_upwind_batched(dcode, int(choice), ptr(s.rows), ptr(s.row_offsets), s.row_count,
+                                                ptr(s.begin), ptr(s.end), ptr(field.interval_cell_offsets),
+                                                ptr(values), ptr(out), ptr(row_inst), ptr(inst_w), ptr(inst_coef),
+                                                ctx.stream), "upwind_batched")
+
+    seg_ok = plan is not None and _segment_kernel_ok(values, out)
+    fixed = "seg" if (seg_ok and sched != "rows") else _group(cells, s.row_count)
+    choice = fixed
+    if sched == "auto":
+        cands = (("seg",) if seg_ok else ()) + autotune.GROUPS
+        key = ("upwind", dcode, True, seg_ok, autotune.bucket(cells), autotune.bucket(cells / max(s.row_count, 1)))
+        tuned = autotune.pick(key, lambda: {c: (lambda c=c: launch(c)) for c in cands})
+        choice = fixed if tuned is None else tuned
+    launch(choice)
     return out

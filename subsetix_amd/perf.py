@@ -18,7 +18,7 @@ from typing import Dict, List
 import numpy as np
 import torch
 
-from . import _core, workloads
+from . import _core, autotune, workloads
 from .expressions import IntervalSet
 
 HBM_PEAK_GBS = 8000.0
@@ -57,9 +57,10 @@ def setalgebra_batched(batch: int, steps: int, warmup: int, dev, group=None) -> 
                                capacity=cap, buffers=bufs[op], group=group)[3]
 
     totals = {}
-    for op in range(3):
-        ctx = run_op(op)
-        (totals[op],) = _core.read_counts(ctx, ctx.words[1])
+    with autotune.tuning(group is None):
+        for op in range(3):
+            ctx = run_op(op)
+            (totals[op],) = _core.read_counts(ctx, ctx.words[1])
     for _ in range(warmup):
         for op in range(3):
             run_op(op)
@@ -87,8 +88,20 @@ def setalgebra_batched(batch: int, steps: int, warmup: int, dev, group=None) -> 
         "kernel_gbs": {k: byts[op] / (kms[op] * 1e-3) / 1e9 for k, op in (("union", 0), ("intersection", 1),
                                                                            ("difference", 2))},
         "algorithmic_bytes": {"union": byts[0], "intersection": byts[1], "difference": byts[2]},
+        "autotune": autotune_report(("setop",)) if group is None else None,
         "host_pair": pairs[0],
     }
+
+
+def autotune_report(families) -> Dict:
+    """The autotuner's table for some families: choice and candidate times
+    (microseconds) per shape bucket."""
+    out = {}
+    for key, e in autotune.default().table.items():
+        if key[0] in families:
+            out["/".join(str(k) for k in key)] = {"choice": str(e.choice),
+                                                  "us": {k: round(v, 1) for k, v in e.times_us.items()}}
+    return out
 
 
 def setalgebra_plain(pair, reps: int, dev) -> Dict:
@@ -155,6 +168,7 @@ def large_stencil(steps: int, dev, n: int = 8192, seed: int = 3) -> Dict:
             fv0 = torch.from_numpy(rng.random(comp.fine_field.values.numel()).astype(np.float32)).to(dev)
         comp.set_values(cv0, fv0)
         m0 = comp.mass()
+        comp.prepare(tune=True)
         for _ in range(3):
             comp.step()
         comp.set_values(cv0, fv0)
@@ -219,6 +233,7 @@ def large_stencil(steps: int, dev, n: int = 8192, seed: int = 3) -> Dict:
     kms = s.elapsed_time(e) / reps
     byts = 8 * f.values.numel() + 16 * f.interval_set.interval_count + 4 * f.interval_set.row_count
     res["upwind_fine_kernel"] = {"ms": kms, "algorithmic_bytes": byts, "gbs": byts / (kms * 1e-3) / 1e9}
+    res["autotune"] = autotune_report(("upwind", "transfer"))
     del runs
     return res
 
@@ -235,7 +250,7 @@ def batched_amr(count: int, steps: int, dev, *, plain_sample: int = 16, plain_st
     from .batch import BatchConfig, BatchedAMR
 
     insts = workloads.batched_instances(count=count, seed=seed)
-    cfg = BatchConfig(levels=3, regrid_every=8, threshold=(0.02, 0.02), grading=1)
+    cfg = BatchConfig(levels=3, regrid_every=8, threshold=(0.02, 0.02), grading=1, autotune=True)
     k = cfg.regrid_every
     t0 = time.perf_counter()
     eng = BatchedAMR(insts, config=cfg, device=dev)
@@ -309,7 +324,18 @@ def batched_amr(count: int, steps: int, dev, *, plain_sample: int = 16, plain_st
                     "desc": "one instance at a time, eager launches, host synchronisation every step "
                             "(regrids included, every 8 steps)"}
     res["batched_vs_plain"] = res["cell_updates_per_s"] / res["plain"]["cell_updates_per_s"]
+    res["autotune"] = {"step": autotune_report(("upwind", "transfer")),
+                       "choices": _choice_counts(("setop", "grad_tag", "upwind", "transfer"))}
     return res
+
+
+def _choice_counts(families) -> Dict:
+    out = {}
+    for key, e in autotune.default().table.items():
+        if key[0] in families:
+            d = out.setdefault(key[0], {})
+            d[str(e.choice)] = d.get(str(e.choice), 0) + 1
+    return out
 
 
 def plain_two_level(dev, n: int = 128, steps: int = 100, regrid_every: int = 10) -> Dict:

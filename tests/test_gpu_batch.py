@@ -109,6 +109,23 @@ def test_batched_upwind_matches_single_instance(torch_mod):
         off += n * n
 
 
+def test_stacked_level0_gradient_tag_segments_equal_rows(torch_mod):
+    """The regrid tag of the stacked level 0 through the level-0 segment
+    table (per-instance widths) equals the row-group kernel."""
+    from subsetix_amd.batch import BatchConfig, BatchedAMR
+    from subsetix_amd.levels import gradient_tag
+
+    eng = BatchedAMR(_instances(), config=BatchConfig(levels=1, use_graph=False))
+    eng.run(3)
+    f = eng._fields(eng.cur)[0]
+    for t in (0.005, 0.02):
+        a = to_host(gradient_tag(f, t, plan=eng._plan0))
+        b = to_host(gradient_tag(f, t))
+        for name in ("rows", "off", "beg", "end"):
+            np.testing.assert_array_equal(getattr(a, name), getattr(b, name), err_msg=name)
+        assert a.n > 0
+
+
 def _dilate8(cells):
     out = set(cells)
     for (y, x) in cells:

@@ -55,7 +55,7 @@ def test_prolong_restrict_into(sx, dt, schedule, touching):
     try:
         _prolong_restrict_case(sx, dt, touching)
     finally:
-        levels.set_transfer_schedule("segments")
+        levels.set_transfer_schedule("auto")
 
 
 def _prolong_restrict_case(sx, dt, touching):
@@ -133,15 +133,26 @@ def _grad_tag_model(s, vals, t):
     return tagged
 
 
-def test_gradient_tag_sparse_and_dense(sx):
+@pytest.mark.parametrize("schedule", ["rows", "seg"])
+def test_gradient_tag_sparse_and_dense(sx, schedule):
+    """Per-cell model (sparse set, touching intervals) and the dense oracle;
+    'seg' runs the segment-table kernel (bit masks), 'rows' the row-group
+    kernel."""
     import torch
 
+    from subsetix_amd.interval_stencil import build_stencil_plan
     from subsetix_amd.levels import gradient_tag
 
+    def tag(s, vals, t, width):
+        f = _field(sx, s, vals, torch.float32)
+        plan = build_stencil_plan(f.interval_set, f.interval_cell_offsets, width) if schedule == "seg" else None
+        return to_host(gradient_tag(f, t, plan=plan))
+
+    print("grad tag", schedule, flush=True)
     rng = np.random.default_rng(9)
     s = random_canonical(rng, 40, 80, mean_n=4, rows=np.arange(40), touching=True)
     v = rng.random(int(ox.cell_offsets(s)[-1])).astype(np.float32)
-    got = to_host(gradient_tag(_field(sx, s, v, torch.float32), 0.3))
+    got = tag(s, v, 0.3, 80)
     assert ox.cells(got) == _grad_tag_model(s, v, 0.3)
     # runs are maximal within adjacency
     for row, ivs in canonical_rows(got).items():
@@ -151,8 +162,53 @@ def test_gradient_tag_sparse_and_dense(sx):
     w, h = 70, 33
     d = rng.random((h, w)).astype(np.float32)
     dense = ox.full(w, h)
-    got = to_host(gradient_tag(_field(sx, dense, d.reshape(-1), torch.float32), 0.4))
+    got = tag(dense, d.reshape(-1), 0.4, w)
     assert_set_equal(got, ox.gradient_tag(d, w, h, 0.4))
+
+
+@pytest.mark.parametrize("dt", ["f32", "f64"])
+def test_gradient_tag_seg_equals_rows(sx, dt):
+    """Segment kernel == row-group kernel, bit for bit: several cell tiles,
+    sparse rows with gaps, touching intervals, long tagged runs crossing
+    32-cell words and tiles, an empty field."""
+    import torch
+
+    from subsetix_amd.interval_stencil import build_stencil_plan
+    from subsetix_amd.levels import gradient_tag
+
+    tdt = torch.float32 if dt == "f32" else torch.float64
+    npdt = np.float32 if dt == "f32" else np.float64
+    rng = np.random.default_rng(23)
+    rows = np.sort(rng.choice(400, 300, replace=False))
+    cases = [(random_canonical(rng, 300, 700, mean_n=12, rows=rows, touching=True), 700),
+             (ox.full(3000, 7), 3000)]
+    for s, w in cases:
+        n = int(ox.cell_offsets(s)[-1])
+        x = np.linspace(0.0, 40.0, n)
+        vals = (np.sin(x) * (rng.random(n) < 0.8) + 0.01 * rng.random(n)).astype(npdt)
+        f = _field(sx, s, vals, tdt)
+        plan = build_stencil_plan(f.interval_set, f.interval_cell_offsets, w)
+        for t in (0.02, 0.3):
+            print("grad tag seg vs rows", dt, n, t, flush=True)
+            a = to_host(gradient_tag(f, t, plan=plan))
+            b = to_host(gradient_tag(f, t))
+            for name in ("rows", "off", "beg", "end"):
+                np.testing.assert_array_equal(getattr(a, name), getattr(b, name), err_msg=name)
+            assert a.n > 0
+
+
+def test_gradient_tag_seg_rejects_cells_outside_domain(sx):
+    import torch
+
+    from subsetix_amd._lib import StxError
+    from subsetix_amd.interval_stencil import build_stencil_plan
+    from subsetix_amd.levels import gradient_tag
+
+    s = ox.make([0, 1, 2], [0, 5], [40, 60])
+    f = _field(sx, s, np.ones(95, np.float32), torch.float32)
+    plan = build_stencil_plan(f.interval_set, f.interval_cell_offsets, 50)  # row 1 reaches x = 59
+    with pytest.raises(StxError, match="domain"):
+        gradient_tag(f, 0.1, plan=plan)
 
 
 def _oracle_composite_step(c_set, cv, ext, fv, ghost, refine, W, H, vel, dt, dtype):

@@ -342,4 +342,4 @@ def test_stencil_schedules_vs_oracle(sx, schedule):
                 got64 = st.step_upwind_interval(field, compute_dtype=torch.float64, **kw).cpu().numpy()
                 np.testing.assert_array_equal(got64, ox.upwind(s, u.astype(np.float64), **kw))
     finally:
-        st.set_stencil_schedule("tiled")
+        st.set_stencil_schedule("auto")
