@@ -268,7 +268,7 @@ __device__ __forceinline__ bool key_a_first(int32_t akb, int32_t bkb, int32_t bk
 constexpr int kFused = 0, kCount = 1, kWrite = 2;
 
 template <int OP, int MODE, bool XF>
-__global__ __launch_bounds__(kTB) void setop_tile_kernel(Operand A, Operand B, int64_t U, int32_t RPT, uint32_t ntiles,
+__global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(8))) void setop_tile_kernel(Operand A, Operand B, int64_t U, int32_t RPT, uint32_t ntiles,
                                                             int32_t* __restrict__ out_off, int32_t* __restrict__ ob,
                                                             int32_t* __restrict__ oe, int64_t capacity,
                                                             int32_t* __restrict__ d_total, uint32_t* status,
@@ -485,6 +485,41 @@ __global__ __launch_bounds__(kTB) void setop_tile_kernel(Operand A, Operand B, i
         if (tid == 0) state_store(&lb.state[tile], epoch, 1u, (uint32_t)tile_total);
         return;
     }
+    if (fits) {
+        // stage the tile's output intervals in LDS (the input stage is free
+        // once every walk is done; emissions <= positions <= kPCAP) before
+        // the look-back, so that the global writes are one coalesced copy
+        __syncthreads();
+        int32_t w = my_base;
+        if (OP == 0) {
+            // starts write begins; a start that is not the first of its row
+            // closes the previous run; the last position of a row closes the
+            // row's last run.
+            int32_t pm = carry.v;
+            const int32_t p0 = tid * kVT;
+#pragma unroll
+            for (int v = 0; v < kVT; ++v) {
+                if (p0 + v >= P) break;
+                if (rowstart_mask & (1u << v)) pm = INT_MIN;
+                const bool st = start_mask & (1u << v);
+                if (st) {
+                    s_k[w].x = rb[v];
+                    if (pm != INT_MIN) s_k[w - 1].y = pm;
+                    ++w;
+                }
+                if (alive_mask & (1u << v)) pm = pm > re[v] ? pm : re[v];
+                if ((rowend_mask & (1u << v)) && pm != INT_MIN) s_k[w - 1].y = pm;
+            }
+        } else {
+#pragma unroll
+            for (int v = 0; v < kVT; ++v) {
+                if (emit_mask & (1u << v)) {
+                    s_k[w] = make_int2(rb[v], re[v]);
+                    ++w;
+                }
+            }
+        }
+    }
     if (MODE == kFused && wave_id() == 0) {
         uint32_t e = lookback_exclusive(lb, tile, (uint32_t)tile_total, epoch, status, spin_limit);
         if (lane_id() == 0) s_excl = e;
@@ -518,35 +553,10 @@ __global__ __launch_bounds__(kTB) void setop_tile_kernel(Operand A, Operand B, i
     if (excl + tile_total > capacity) {
         if (tid == 0 && tile_total > 0) atomicOr(status, STX_FLAG_CAPACITY);
     } else if (fits) {
-        int64_t w = excl + my_base;
-        if (OP == 0) {
-            // starts write begins; a start that is not the first of its row
-            // closes the previous run; the last position of a row closes the
-            // row's last run.
-            int32_t pm = carry.v;
-            const int32_t p0 = tid * kVT;
-#pragma unroll
-            for (int v = 0; v < kVT; ++v) {
-                if (p0 + v >= P) break;
-                if (rowstart_mask & (1u << v)) pm = INT_MIN;
-                const bool st = start_mask & (1u << v);
-                if (st) {
-                    ob[w] = rb[v];
-                    if (pm != INT_MIN) oe[w - 1] = pm;
-                    ++w;
-                }
-                if (alive_mask & (1u << v)) pm = pm > re[v] ? pm : re[v];
-                if ((rowend_mask & (1u << v)) && pm != INT_MIN) oe[w - 1] = pm;
-            }
-        } else {
-#pragma unroll
-            for (int v = 0; v < kVT; ++v) {
-                if (emit_mask & (1u << v)) {
-                    ob[w] = rb[v];
-                    oe[w] = re[v];
-                    ++w;
-                }
-            }
+        for (int k = tid; k < tile_total; k += kTB) {
+            const int2 o = s_k[k];
+            ob[excl + k] = o.x;
+            oe[excl + k] = o.y;
         }
     } else {
         const int wave = wave_id();
