@@ -11,9 +11,10 @@ their mean per row), so one tuning serves a regrid cadence or a stream of
 similar queries.
 
 Families and candidates:
-  setop      fused tile kernel (-1), two-pass tile kernel (-2), one lane
-             group of 4 / 16 / 64 lanes per row (4 / 16 / 64); operands
-             with row maps or transforms have the lane-group candidates only
+  setop      tile kernel 512 x 8 single pass (-1) or two passes (-2),
+             128 x 16 (-3), 64 x 32 (-4) (threads x positions per thread),
+             one lane group of 4 / 16 / 64 lanes per row (4 / 16 / 64);
+             operands with row maps have the lane-group candidates only
   upwind     segment tiles ("seg") and lane groups of 4 / 16 / 64 per row
   transfer   segment tables ("seg", ratio 2) and lane groups per target row
   grad_tag   lane groups of 4 / 16 / 64 per row

@@ -205,11 +205,30 @@ struct SetopRow {
 // the block, the tile offset comes from the decoupled look-back, and the
 // intervals are written in tile order (contiguous).  A tile whose rows hold
 // more than PCAP intervals falls back to one wavefront per row.
-constexpr int kTB = 512;              // threads of a tile block
-constexpr int kTW = kTB / STX_WAVE;   // its waves
-constexpr int kVT = 8;
-constexpr int kPCAP = kTB * kVT;
-constexpr int kMaxRPT = 256;
+// Tile shapes: TB threads, VT merged positions per thread (capacity TB * VT
+// positions per tile).  512 x 8 keeps 4 tiles (32 waves) on a CU; the
+// narrower shapes amortise each thread's searches and the block scans over
+// more positions with fewer barriers, at fewer waves per CU.
+template <int VT>
+struct MaskOf {
+    typedef uint32_t type;
+};
+template <>
+struct MaskOf<8> {
+    typedef uint8_t type;
+};
+template <>
+struct MaskOf<16> {
+    typedef uint16_t type;
+};
+template <int TB>
+constexpr int max_rows_per_tile() {
+    return TB < 256 ? TB : 256;  // the fallback scans one row count per thread
+}
+template <int VT>
+constexpr int tile_waves_per_eu() {
+    return VT <= 8 ? 8 : (VT <= 16 ? 5 : 3);
+}
 
 struct SegMax {
     int32_t v;
@@ -224,7 +243,7 @@ __device__ __forceinline__ SegMax seg_combine(SegMax a, SegMax b) {  // a before
 }
 
 // Block-wide exclusive segmented max-scan (one element per thread).
-__device__ __forceinline__ SegMax block_seg_max_exclusive(SegMax x, SegMax* lds /* >= kTW */) {
+__device__ __forceinline__ SegMax block_seg_max_exclusive(SegMax x, SegMax* lds /* >= waves of the block */) {
     const int lane = lane_id(), wave = wave_id();
     SegMax inc = x;
 #pragma unroll
@@ -267,20 +286,23 @@ __device__ __forceinline__ bool key_a_first(int32_t akb, int32_t bkb, int32_t bk
 // never wait on each other; every scratch word stays epoch-tagged.
 constexpr int kFused = 0, kCount = 1, kWrite = 2;
 
-template <int OP, int MODE, bool XF>
-__global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(8))) void setop_tile_kernel(Operand A, Operand B, int64_t U, int32_t RPT, uint32_t ntiles,
+template <int OP, int MODE, bool XF, int TB, int VT>
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(tile_waves_per_eu<VT>()))) void setop_tile_kernel(Operand A, Operand B, int64_t U, int32_t RPT, uint32_t ntiles,
                                                             int32_t* __restrict__ out_off, int32_t* __restrict__ ob,
                                                             int32_t* __restrict__ oe, int64_t capacity,
                                                             int32_t* __restrict__ d_total, uint32_t* status,
                                                             void* scratch, uint32_t epoch, int64_t spin_limit) {
-    // LDS kept under 20 KiB so 8 blocks (32 waves) fit on a CU
+    constexpr int kTW = TB / STX_WAVE;
+    constexpr int kPCAP = TB * VT;
+    constexpr int kMaxRPT = max_rows_per_tile<TB>();
+    typedef typename MaskOf<VT>::type MaskT;
     __shared__ int2 s_k[kPCAP];  // (begin, end) of the tile's A then B intervals
     __shared__ int32_t s_aS[kMaxRPT + 1];
     __shared__ int32_t s_bS[kMaxRPT + 1];
     // per thread: emissions before its first position, and its emission mask
     // (fallback path: per-row counts, then their exclusive scan)
-    __shared__ int32_t s_tbase[kTB + 1];
-    __shared__ uint8_t s_tmask[kTB];
+    __shared__ int32_t s_tbase[TB + 1];
+    __shared__ MaskT s_tmask[TB];
     __shared__ int32_t s_scan[17];
     __shared__ SegMax s_seg[kTW];
     __shared__ uint32_t s_excl;
@@ -295,7 +317,7 @@ __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(8))) void s
     // exists; the trailing tile (nr == 0 when RPT divides U) then sees P = 0.
     const int32_t a_base = A.off[row0];
     const int32_t b_base = B.off ? B.off[row0] : 0;
-    for (int r = tid; r <= nr; r += kTB) {
+    for (int r = tid; r <= nr; r += TB) {
         int32_t as = A.off[row0 + r] - a_base;
         int32_t bs = B.off ? B.off[row0 + r] - b_base : 0;
         s_aS[r] = as;
@@ -308,21 +330,21 @@ __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(8))) void s
 
     int32_t tile_total = 0;
     // per-position results (registers)
-    int32_t rb[kVT], re[kVT];
+    int32_t rb[VT], re[VT];
     uint32_t emit_mask = 0, start_mask = 0, alive_mask = 0, rowstart_mask = 0, rowend_mask = 0;
     SegMax carry{INT_MIN, 0};
     int32_t my_base = 0;
 
     if (fits) {
         if (XF) {
-            for (int k = tid; k < nA; k += kTB) s_k[k] = make_int2(keyb(A, a_base + k), keye(A, a_base + k));
-            for (int k = tid; k < nB; k += kTB) s_k[nA + k] = make_int2(keyb(B, b_base + k), keye(B, b_base + k));
+            for (int k = tid; k < nA; k += TB) s_k[k] = make_int2(keyb(A, a_base + k), keye(A, a_base + k));
+            for (int k = tid; k < nB; k += TB) s_k[nA + k] = make_int2(keyb(B, b_base + k), keye(B, b_base + k));
         } else {
-            for (int k = tid; k < nA; k += kTB) s_k[k] = make_int2(A.beg[a_base + k], A.end[a_base + k]);
-            for (int k = tid; k < nB; k += kTB) s_k[nA + k] = make_int2(B.beg[b_base + k], B.end[b_base + k]);
+            for (int k = tid; k < nA; k += TB) s_k[k] = make_int2(A.beg[a_base + k], A.end[a_base + k]);
+            for (int k = tid; k < nB; k += TB) s_k[nA + k] = make_int2(B.beg[b_base + k], B.end[b_base + k]);
         }
         __syncthreads();
-        const int32_t p0 = tid * kVT;
+        const int32_t p0 = tid * VT;
         if (p0 < P) {
             // row of p0
             int lo = 0, hi = nr;
@@ -367,7 +389,7 @@ __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(8))) void s
                 pBy = j >= 1 ? t3.y : INT_MIN;
             }
 #pragma unroll
-            for (int v = 0; v < kVT; ++v) {
+            for (int v = 0; v < VT; ++v) {
                 const int32_t p = p0 + v;
                 rb[v] = 0;
                 re[v] = INT_MIN;
@@ -444,14 +466,14 @@ __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(8))) void s
             // thread aggregate: max end since the last row start in the thread
             SegMax agg{INT_MIN, 0};
 #pragma unroll
-            for (int v = 0; v < kVT; ++v) {
+            for (int v = 0; v < VT; ++v) {
                 if (rowstart_mask & (1u << v)) agg = SegMax{INT_MIN, 1};
                 if (alive_mask & (1u << v)) agg.v = agg.v > re[v] ? agg.v : re[v];
             }
             carry = block_seg_max_exclusive(agg, s_seg);
             int32_t pm = carry.v;
 #pragma unroll
-            for (int v = 0; v < kVT; ++v) {
+            for (int v = 0; v < VT; ++v) {
                 if (rowstart_mask & (1u << v)) pm = INT_MIN;
                 if ((alive_mask & (1u << v)) && rb[v] > pm) start_mask |= 1u << v;
                 if (alive_mask & (1u << v)) pm = pm > re[v] ? pm : re[v];
@@ -461,7 +483,7 @@ __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(8))) void s
         const int32_t my_count = __popc(emit_mask);
         my_base = block_exclusive_scan<int32_t>(my_count, s_scan, &tile_total);
         s_tbase[tid] = my_base;
-        s_tmask[tid] = (uint8_t)emit_mask;
+        s_tmask[tid] = (MaskT)emit_mask;
     } else {
         // fallback: one wavefront per row, counts into s_tbase[r]
         const int wave = wave_id();
@@ -475,7 +497,7 @@ __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(8))) void s
         __syncthreads();
         int32_t v = tid < nr ? s_tbase[tid] : 0;
         int32_t ex = 0;
-        // nr <= kMaxRPT <= kTB
+        // nr <= kMaxRPT <= TB
         ex = block_exclusive_scan<int32_t>(v, s_scan, &tile_total);
         __syncthreads();
         if (tid < nr) s_tbase[tid] = ex;
@@ -496,9 +518,9 @@ __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(8))) void s
             // closes the previous run; the last position of a row closes the
             // row's last run.
             int32_t pm = carry.v;
-            const int32_t p0 = tid * kVT;
+            const int32_t p0 = tid * VT;
 #pragma unroll
-            for (int v = 0; v < kVT; ++v) {
+            for (int v = 0; v < VT; ++v) {
                 if (p0 + v >= P) break;
                 if (rowstart_mask & (1u << v)) pm = INT_MIN;
                 const bool st = start_mask & (1u << v);
@@ -512,7 +534,7 @@ __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(8))) void s
             }
         } else {
 #pragma unroll
-            for (int v = 0; v < kVT; ++v) {
+            for (int v = 0; v < VT; ++v) {
                 if (emit_mask & (1u << v)) {
                     s_k[w] = make_int2(rb[v], re[v]);
                     ++w;
@@ -528,7 +550,7 @@ __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(8))) void s
     const int64_t excl =
         MODE == kFused ? (int64_t)s_excl : (tile == 0 ? 0 : (int64_t)(uint32_t)state_load(&lb.state[tile - 1]));
     // row offsets
-    for (int r = tid; r <= nr; r += kTB) {
+    for (int r = tid; r <= nr; r += TB) {
         if (r == nr && row0 + nr != U) break;
         int32_t local;
         if (fits) {
@@ -537,8 +559,8 @@ __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(8))) void s
             if (q >= P) {
                 local = tile_total;
             } else {
-                const int t = q / kVT;
-                local = s_tbase[t] + __popc((uint32_t)s_tmask[t] & ((1u << (q % kVT)) - 1u));
+                const int t = q / VT;
+                local = s_tbase[t] + __popc((uint32_t)s_tmask[t] & ((1u << (q % VT)) - 1u));
             }
         } else {
             local = s_tbase[r];
@@ -553,7 +575,7 @@ __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(8))) void s
     if (excl + tile_total > capacity) {
         if (tid == 0 && tile_total > 0) atomicOr(status, STX_FLAG_CAPACITY);
     } else if (fits) {
-        for (int k = tid; k < tile_total; k += kTB) {
+        for (int k = tid; k < tile_total; k += TB) {
             const int2 o = s_k[k];
             ob[excl + k] = o.x;
             oe[excl + k] = o.y;
@@ -721,44 +743,45 @@ __global__ void add_i32_kernel(int32_t* p, int64_t n, int32_t v) {
 
 // Rows per tile for the LDS tile kernel: about 3/4 of the tile capacity at
 // the mean row length (the row-bucket schedule of the tile path).
+template <int TB, int VT>
 static int32_t setop_tile_rows(int64_t elems, int64_t rows) {
     double avg = rows > 0 ? (double)elems / (double)rows : 1.0;
     if (avg < 1.0) avg = 1.0;
-    int64_t r = (int64_t)(0.75 * kPCAP / avg);
+    int64_t r = (int64_t)(0.75 * TB * VT / avg);
     if (r < 1) r = 1;
-    if (r > kMaxRPT) r = kMaxRPT;
+    if (r > max_rows_per_tile<TB>()) r = max_rows_per_tile<TB>();
     return (int32_t)r;
 }
 
 constexpr uint32_t kTwoPassTiles = 0xffffffffu;  // measured slower than one pass at every size tried
 
-template <int OP, int MODE>
+template <int OP, int MODE, int TB, int VT>
 static void launch_tile_op(uint32_t ntiles, hipStream_t stream, const Operand& A, const Operand& B, int64_t U, int32_t rpt,
                            int32_t* out_off, int32_t* ob, int32_t* oe, int64_t capacity, int32_t* d_total,
                            uint32_t* status, void* scratch, uint32_t epoch, int64_t spin) {
     // operands read as stored (no transform on load): the plain kernel
     const bool xf = A.xf.mode != XF_NONE || (B.off != nullptr && B.xf.mode != XF_NONE);
     if (xf)
-        hipLaunchKernelGGL((setop_tile_kernel<OP, MODE, true>), dim3(ntiles), dim3(kTB), 0, stream, A, B, U, rpt,
+        hipLaunchKernelGGL((setop_tile_kernel<OP, MODE, true, TB, VT>), dim3(ntiles), dim3(TB), 0, stream, A, B, U, rpt,
                            ntiles, out_off, ob, oe, capacity, d_total, status, scratch, epoch, spin);
     else
-        hipLaunchKernelGGL((setop_tile_kernel<OP, MODE, false>), dim3(ntiles), dim3(kTB), 0, stream, A, B, U, rpt,
-                           ntiles, out_off, ob, oe, capacity, d_total, status, scratch, epoch, spin);
+        hipLaunchKernelGGL((setop_tile_kernel<OP, MODE, false, TB, VT>), dim3(ntiles), dim3(TB), 0, stream, A, B, U,
+                           rpt, ntiles, out_off, ob, oe, capacity, d_total, status, scratch, epoch, spin);
 }
 
-template <int MODE>
+template <int MODE, int TB = 512, int VT = 8>
 static void launch_tile(int32_t op, uint32_t ntiles, hipStream_t stream, const Operand& A, const Operand& B, int64_t U,
                         int32_t rpt, int32_t* out_off, int32_t* ob, int32_t* oe, int64_t capacity, int32_t* d_total,
                         uint32_t* status, void* scratch, uint32_t epoch, int64_t spin) {
     if (op == 0)
-        launch_tile_op<0, MODE>(ntiles, stream, A, B, U, rpt, out_off, ob, oe, capacity, d_total, status, scratch, epoch,
-                                spin);
+        launch_tile_op<0, MODE, TB, VT>(ntiles, stream, A, B, U, rpt, out_off, ob, oe, capacity, d_total, status,
+                                        scratch, epoch, spin);
     else if (op == 1)
-        launch_tile_op<1, MODE>(ntiles, stream, A, B, U, rpt, out_off, ob, oe, capacity, d_total, status, scratch, epoch,
-                                spin);
+        launch_tile_op<1, MODE, TB, VT>(ntiles, stream, A, B, U, rpt, out_off, ob, oe, capacity, d_total, status,
+                                        scratch, epoch, spin);
     else
-        launch_tile_op<2, MODE>(ntiles, stream, A, B, U, rpt, out_off, ob, oe, capacity, d_total, status, scratch, epoch,
-                                spin);
+        launch_tile_op<2, MODE, TB, VT>(ntiles, stream, A, B, U, rpt, out_off, ob, oe, capacity, d_total, status,
+                                        scratch, epoch, spin);
 }
 
 template <int G>
@@ -811,12 +834,27 @@ int32_t stx_setop(int32_t op, const int32_t* a_beg, const int32_t* a_end, const 
         return stx_fail(STX_EINVAL, "setop: coarsening ratio must be >= 1");
     const bool tile_path = a_map == nullptr && b_map == nullptr && d_nrows == nullptr && group <= 0;
     if (tile_path) {
-        const int32_t rpt = setop_tile_rows(a_n + (b_off ? b_n : 0), nrows_bound);
+        const int64_t elems = a_n + (b_off ? b_n : 0);
+        const int64_t spin = stx_spin_limit();
+        if (group == -3 || group == -4) {
+            // narrower tile shapes, single pass: -3 = 128 x 16, -4 = 64 x 32
+            const int32_t rpt = group == -3 ? setop_tile_rows<128, 16>(elems, nrows_bound)
+                                            : setop_tile_rows<64, 32>(elems, nrows_bound);
+            const uint32_t ntiles = (uint32_t)((nrows_bound + rpt) / rpt);
+            if (lookback_bytes(ntiles) > scratch_bytes) return stx_fail(STX_EINVAL, "setop: scratch too small");
+            if (group == -3)
+                launch_tile<kFused, 128, 16>(op, ntiles, stream, A, B, nrows_bound, rpt, out_off, out_beg, out_end,
+                                             capacity, d_total, status, scratch, epoch, spin);
+            else
+                launch_tile<kFused, 64, 32>(op, ntiles, stream, A, B, nrows_bound, rpt, out_off, out_beg, out_end,
+                                            capacity, d_total, status, scratch, epoch, spin);
+            return stx_launch_status("setop_tile_kernel");
+        }
+        const int32_t rpt = setop_tile_rows<512, 8>(elems, nrows_bound);
         const uint32_t ntiles = (uint32_t)((nrows_bound + rpt) / rpt);
         // group 0: automatic (single pass: the two-pass form was slower at
         // every size measured); -1: single pass; -2: two passes
         const bool two_pass = group == -2 || (group == 0 && ntiles >= kTwoPassTiles);
-        const int64_t spin = stx_spin_limit();
         if (lookback_bytes(ntiles) > scratch_bytes) return stx_fail(STX_EINVAL, "setop: scratch too small");
         if (!two_pass) {
             launch_tile<kFused>(op, ntiles, stream, A, B, nrows_bound, rpt, out_off, out_beg, out_end, capacity, d_total,

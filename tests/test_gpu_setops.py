@@ -55,7 +55,7 @@ def test_golden_cell_model(sx, golden):
             assert ox.cells(to_host(got)) == cell_model(spec, cells), spec
 
 
-@pytest.mark.parametrize("group", [None, 4, 16, 64])  # None: LDS tile path
+@pytest.mark.parametrize("group", [None, -2, -3, -4, 4, 16, 64])  # None / negative: LDS tile shapes
 @pytest.mark.parametrize("seed", [0, 1, 2])
 def test_random_vs_oracle(sx, seed, group):
     from subsetix_amd import _core
@@ -197,12 +197,14 @@ def test_row_count_multiple_of_tile_rows(sx):
             assert_set_equal(to_host(_ops(sx)[op](da, db)), ox.setop(op, a, b))
 
 
-@pytest.mark.parametrize("schedule", [-1, -2])
+@pytest.mark.parametrize("schedule", [-1, -2, -3, -4])
 def test_tile_schedules_batched_pairs(sx, schedule):
-    """Single-pass (-1, decoupled look-back) and two-pass (-2, count kernel,
-    prefix kernel, write kernel) tile schedules on 16 stacked full-size pairs
-    (several hundred tiles), bit-exact against the oracle; repeated launches
-    on one scratch buffer alternate the two schedules."""
+    """Tile schedules on 16 stacked full-size pairs (hundreds of tiles),
+    bit-exact against the oracle: single pass 512 x 8 (-1, decoupled
+    look-back), two passes (-2, count kernel, prefix kernel, write kernel),
+    single pass 128 x 16 (-3) and 64 x 32 (-4); repeated launches on one
+    scratch buffer alternate schedules.  Dilation (transforms on load, one
+    operand) runs through the same shape."""
     import torch
 
     from subsetix_amd import _core, workloads
@@ -215,7 +217,7 @@ def test_tile_schedules_batched_pairs(sx, schedule):
     R = da.row_count
     for op, name in ((0, "union"), (1, "intersection"), (2, "difference")):
         exp = ox.setop(name, a, b)
-        for g in (schedule, -3 - schedule, schedule):  # -1 -> -2 -> -1, or -2 -> -1 -> -2
+        for g in (schedule, -2 if schedule == -1 else -1, schedule):
             print("schedule", g, name, flush=True)
             out_off, ob, oe, ctx = _core.setop_raw(op, (da.begin, da.end, da.row_offsets),
                                                    (db.begin, db.end, db.row_offsets), nrows=R, group=g)
@@ -223,3 +225,11 @@ def test_tile_schedules_batched_pairs(sx, schedule):
             np.testing.assert_array_equal(out_off.cpu().numpy(), exp.off)
             np.testing.assert_array_equal(ob[:total].cpu().numpy(), exp.beg)
             np.testing.assert_array_equal(oe[:total].cpu().numpy(), exp.end)
+    small_a = hostset_to_oset(pairs[0][0])
+    _core.set_group_override(schedule)
+    try:
+        for r in (1, 3):
+            print("schedule", schedule, "dilate", r, flush=True)
+            assert_set_equal(to_host(sx.dilate_interval_set(to_device(small_a), r, r)), ox.dilate(small_a, r, r))
+    finally:
+        _core.set_group_override(None)
