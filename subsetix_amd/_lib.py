@@ -35,9 +35,12 @@ _SIGNATURES: Dict[str, Tuple[object, list]] = {
     "stx_setop": (_I32, [_I32, _P, _P, _P, _P, _I32, _I32, _I64, _P, _P, _P, _P, _I32, _I32, _I64, _P, _I64, _I32,
                          _P, _P, _P, _I64, _P, _P, _P, _I64, _I32, _P]),
     "stx_rows_union": (_I32, [_P, _I64, _I32, _P, _I64, _I32, _P, _P, _P, _P, _P, _P, _P, _I64, _I32, _P]),
+    "stx_row_instance": (_I32, [_P, _I64, _P, _I32, _P, _P]),
+    "stx_instance_mass": (_I32, [_I32, _P, _P, _I32, _P, _P, _P]),
 }
 
 _lib = None
+_handle = None
 _lib_lock = threading.Lock()
 
 
@@ -45,10 +48,24 @@ class HipLibraryError(RuntimeError):
     pass
 
 
-def lib() -> ctypes.CDLL:
+class _Library:
+    """The loaded library, exposing only entry points with a registered
+    signature: ctypes would pass the 64-bit device pointers of an
+    unprototyped call as 32-bit ints."""
+
+    def __init__(self, handle: ctypes.CDLL):
+        self._handle = handle
+
+    def __getattr__(self, name: str):
+        if name not in _SIGNATURES:
+            raise HipLibraryError(f"{name} has no registered signature (subsetix_amd._lib.register)")
+        return getattr(self._handle, name)
+
+
+def lib() -> _Library:
     """Load libsubsetix_hip.so (built by ``__graft_entry__.build()`` or
     ``make -C subsetix_amd/csrc``).  Raises if it is missing."""
-    global _lib
+    global _lib, _handle
     if _lib is not None:
         return _lib
     with _lib_lock:
@@ -62,7 +79,8 @@ def lib() -> ctypes.CDLL:
                 fn = getattr(handle, name)
                 fn.restype = res
                 fn.argtypes = args
-            _lib = handle
+            _handle = handle
+            _lib = _Library(handle)
     return _lib
 
 
@@ -70,8 +88,8 @@ def register(name: str, restype, argtypes) -> None:
     """Declare the signature of an exported function (used by sibling
     modules for their own entry points)."""
     _SIGNATURES[name] = (restype, argtypes)
-    if _lib is not None:
-        fn = getattr(_lib, name)
+    if _handle is not None:
+        fn = getattr(_handle, name)
         fn.restype = restype
         fn.argtypes = argtypes
 

@@ -48,8 +48,6 @@ from .levels import gradient_tag, prolong_into, restrict_into, transfer_plan, up
 from .morphology import dilate_interval_set
 from .multilevel import prolong_set, restrict_set
 
-_lib.register("stx_row_instance", _I32, [_P, _I64, _P, _I32, _P, _P])
-_lib.register("stx_instance_mass", _I32, [_I32, _P, _P, _I32, _P, _P, _P])
 
 
 def _cells(s: IntervalSet) -> int:

@@ -5,22 +5,28 @@ expressions.py:446-528).
 
 Requests arrive on a Poisson clock.  The scheduler admits them in arrival
 order while their estimated device bytes fit the HBM budget (a request that
-does not fit waits; later requests do not overtake it).  The server loop then
+does not fit waits; later requests do not overtake it).  The server loop
 keeps every admitted request resident and, per iteration:
 
   * set-algebra queries admitted since the last iteration are stacked row-wise
     into one CSR pair (row ids shifted per query) and answered by ONE fused
     launch per operation (union, intersection, difference); each query's
     result is its slice of the stacked output;
-  * every resident advection job (a two-level composite instance) advances a
-    chunk of steps by replaying its own captured two-step graph; jobs are
-    spread over a few streams so small jobs overlap on the device.
+  * advection jobs (two-level composite instances) admitted since the last
+    cohort form a COHORT: their instances are stacked into one
+    ``StackedComposite`` (one chain of set operations builds the geometry of
+    all of them, one segment table per level, four launches per step for the
+    whole cohort).  A cohort starts at once when the GPU has none running,
+    else when the batching window has passed or it is full.  Cohorts run on
+    separate streams and advance in chunks of captured-graph replays; a chunk
+    stops at the next step count a job asked for, where that job's fields and
+    FP64 mass are copied out.
 
 A job's result does not depend on what else is resident: queries are
-row-independent, and each job owns its fields and tables and runs
-deterministic kernels.  The plain loop (``plain_loop``) runs the same
-requests one at a time with eager launches and a host synchronisation per
-step, the reference-shaped form.
+row-independent, each cohort instance is bit-identical to the instance run
+alone, and every kernel is deterministic.  The plain loop (``plain_loop``)
+runs the same requests one at a time with eager launches and a host
+synchronisation per step, the reference-shaped form.
 """
 
 from __future__ import annotations
@@ -189,6 +195,85 @@ class AdvectionJob:
         return {"mass": self.comp.mass(), "coarse": self.comp.coarse.values, "fine": self.comp.fine_field.values}
 
 
+class Cohort:
+    """Advection jobs advanced together as one StackedComposite on one
+    stream; results are copied out at each job's own step count."""
+
+    def __init__(self, reqs: List[Request], dev, stream: torch.cuda.Stream, use_graph: bool):
+        from .amr2.stacked import StackedComposite
+
+        self.reqs = reqs
+        self.stream = stream
+        self.use_graph = use_graph
+        self.event: Optional[torch.cuda.Event] = None
+        self.step = 0
+        self.steps = [int(r.payload["steps"]) for r in reqs]
+        self.done_steps = sorted(set(self.steps))
+        self.snaps: Dict[int, dict] = {}
+        self.collected = set()
+        with torch.cuda.stream(stream):
+            p0 = reqs[0].payload
+            self.comp = StackedComposite([{"discs": r.payload["discs"], "n_fine": r.payload["n_fine"],
+                                           "velocity": r.payload["velocity"]} for r in reqs],
+                                         dtype=p0["dtype"], device=dev)
+            # initial values U[0, 1) per instance from its seeded generator,
+            # drawn exactly as a job alone draws them
+            for j, r in enumerate(reqs):
+                gen = torch.Generator(device=dev)
+                gen.manual_seed(int(r.payload["seed"]))
+                nc = r.payload["n_fine"] // 2
+                cv = torch.rand(nc * nc, generator=gen, device=dev, dtype=torch.float32)
+                fv = torch.rand(self.comp.ext_cells(j), generator=gen, device=dev, dtype=torch.float32)
+                self.comp.set_instance_values(j, cv, fv)
+            self.comp.finish_init()
+
+    @property
+    def finished(self) -> bool:
+        return len(self.collected) == len(self.reqs)
+
+    def launch(self, chunk: int) -> None:
+        """Up to ``chunk`` steps, stopping at the next requested step count,
+        then copies of the jobs that end there (no host synchronisation)."""
+        def snap():
+            with torch.cuda.stream(self.stream):
+                for j, k in enumerate(self.steps):
+                    if k == self.step and j not in self.snaps:
+                        self.snaps[j] = self.comp.snapshot(j)
+
+        snap()
+        pending = [k for k in self.done_steps if k > self.step]
+        if pending:
+            n = min(chunk, pending[0] - self.step)
+            self.comp.advance(n, self.stream, self.use_graph)
+            self.step += n
+            snap()
+        with torch.cuda.stream(self.stream):
+            self.event = torch.cuda.Event()
+            self.event.record(self.stream)
+
+    def ready(self) -> bool:
+        return self.event is None or self.event.query()
+
+    def collect(self, keep_outputs: bool) -> List[tuple]:
+        """(job index, output) of the jobs whose copies are complete (call
+        when ``ready``)."""
+        out = []
+        with torch.cuda.stream(self.stream):
+            for j, snap in self.snaps.items():
+                if j in self.collected:
+                    continue
+                o = {"mass": float(snap["mass"].item())}
+                if keep_outputs:
+                    o["coarse"] = snap["coarse"].cpu().numpy()
+                    o["fine"] = snap["fine"].cpu().numpy()
+                self.collected.add(j)
+                out.append((j, o))
+        return out
+
+    def cell_updates(self, j: int) -> int:
+        return self.comp.cells_per_step(j) * self.steps[j]
+
+
 def _stack_queries(reqs: List[Request], dev):
     """Row-stack the A and B sets of several queries (16 empty rows between
     queries); returns device sets and each query's row range."""
@@ -273,20 +358,27 @@ class ServeStats:
 
 
 class Server:
-    """Continuous batching server (see the module docstring)."""
+    """Continuous batching server (see the module docstring).  ``window``:
+    seconds an admitted advection job may wait for others to join its cohort
+    while other cohorts run; ``max_cohort``: instances per cohort."""
 
     def __init__(self, *, budget_bytes: int = HBM_BUDGET_BYTES, streams: int = 4, chunk: int = 8,
-                 use_graph: bool = True, device=None, keep_outputs: bool = False):
+                 use_graph: bool = True, device=None, keep_outputs: bool = False, window: float = 2e-3,
+                 max_cohort: int = 64):
         self.dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
         self.sched = Scheduler(budget_bytes)
         self.streams = [torch.cuda.Stream(self.dev) for _ in range(max(1, streams))]
-        self.chunk = int(chunk) + (int(chunk) % 2)  # even: graphs replay pairs of steps
+        self.chunk = max(1, int(chunk))
         self.use_graph = use_graph
         self.keep_outputs = keep_outputs
+        self.window = float(window)
+        self.max_cohort = max(1, int(max_cohort))
 
     def run(self, trace: List[Request]) -> ServeStats:
         pending = deque(sorted(trace, key=lambda r: r.arrival))
-        active: List[AdvectionJob] = []
+        cohorts: List[Cohort] = []
+        waiting: List[Request] = []  # admitted advection jobs without a cohort yet
+        wait_since = 0.0
         admitted_at: Dict[int, float] = {}
         results: List[Result] = []
         cell_updates = 0
@@ -299,15 +391,7 @@ class Server:
         t0 = time.perf_counter()
         now = lambda: time.perf_counter() - t0
 
-        def launch(job: AdvectionJob) -> int:
-            job.stream.wait_stream(main)
-            with torch.cuda.stream(job.stream):
-                n = job.advance(self.chunk)
-                job.event = torch.cuda.Event()
-                job.event.record(job.stream)
-            return n
-
-        while pending or active or self.sched.waiting:
+        while pending or cohorts or waiting or self.sched.waiting:
             t = now()
             while pending and pending[0].arrival <= t:
                 self.sched.offer(pending.popleft())
@@ -318,14 +402,9 @@ class Server:
                 if r.kind == "query":
                     queries.append(r)
                 else:
-                    s = self.streams[rr % len(self.streams)]
-                    rr += 1
-                    ts = time.perf_counter()
-                    job = AdvectionJob(r, self.dev, s, self.use_graph)
-                    setup += time.perf_counter() - ts
-                    cell_updates += launch(job) * job.cells_per_step  # first chunk right away
-                    active.append(job)
-                    progressed = True
+                    if not waiting:
+                        wait_since = now()
+                    waiting.append(r)
             if queries:
                 iters += 1
                 outs = run_queries(queries, self.dev)
@@ -336,29 +415,44 @@ class Server:
                                           o if self.keep_outputs else {}))
                     self.sched.release(r)
                 progressed = True
-            # jobs whose last chunk finished: next chunk, or completion
-            still = []
-            for job in active:
-                if job.event is not None and not job.event.query():
-                    still.append(job)
-                    continue
-                if job.remaining > 0:
-                    iters += 1
-                    cell_updates += launch(job) * job.cells_per_step
-                    still.append(job)
-                else:
-                    with torch.cuda.stream(job.stream):
-                        out = job.output()
-                        out = ({"mass": out["mass"], "coarse": out["coarse"].cpu().numpy(),
-                                "fine": out["fine"].cpu().numpy()} if self.keep_outputs else {"mass": out["mass"]})
-                    results.append(Result(job.req.rid, "advect", job.req.arrival, admitted_at[job.req.rid],
-                                          now(), out))
-                    self.sched.release(job.req)
+            if waiting and (not cohorts or len(waiting) >= self.max_cohort or now() - wait_since >= self.window):
+                by_dtype: Dict[torch.dtype, List[Request]] = {}
+                for r in waiting[: self.max_cohort]:
+                    by_dtype.setdefault(r.payload["dtype"], []).append(r)
+                waiting = waiting[self.max_cohort:]
+                wait_since = now()
+                for reqs in by_dtype.values():
+                    s = self.streams[rr % len(self.streams)]
+                    rr += 1
+                    s.wait_stream(main)
+                    ts = time.perf_counter()
+                    c = Cohort(reqs, self.dev, s, self.use_graph)
+                    setup += time.perf_counter() - ts
+                    c.launch(self.chunk)
+                    cohorts.append(c)
                 progressed = True
-            active = still
+            # cohorts whose last chunk finished: results, next chunk, retire
+            still = []
+            for c in cohorts:
+                if not c.ready():
+                    still.append(c)
+                    continue
+                for j, out in c.collect(self.keep_outputs):
+                    r = c.reqs[j]
+                    cell_updates += c.cell_updates(j)
+                    results.append(Result(r.rid, "advect", r.arrival, admitted_at[r.rid], now(), out))
+                if c.finished:
+                    for r in c.reqs:
+                        self.sched.release(r)
+                else:
+                    iters += 1
+                    c.launch(self.chunk)
+                    still.append(c)
+                progressed = True
+            cohorts = still
             if not progressed:
                 nxt = pending[0].arrival - now() if pending else 0.0
-                time.sleep(min(max(nxt, 0.0), 2e-4) if (active or nxt > 0) else 0.0)
+                time.sleep(min(max(nxt, 0.0), 2e-4) if (cohorts or waiting or nxt > 0) else 0.0)
         torch.cuda.synchronize(self.dev)
         wall = now()
         st = ServeStats(sorted(results, key=lambda r: r.rid), wall, cell_updates, qints, iters, self.sched.peak)
