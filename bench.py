@@ -55,7 +55,7 @@ def parse():
     p.add_argument("--amr-instances", type=int, default=512)
     p.add_argument("--amr-steps", type=int, default=200)
     p.add_argument("--serve-rates", default="100,1000,10000", help="Poisson arrival rates (requests/s) for the serve section")
-    p.add_argument("--serve-requests", type=int, default=64)
+    p.add_argument("--serve-requests", type=int, default=256)
     p.add_argument("--plain-reps", type=int, default=50, help="repetitions of the plain single-pair form")
     return p.parse_args()
 

@@ -169,7 +169,8 @@ int32_t stx_upwind(int32_t dtype, int32_t group, const int32_t* rows, const int3
  * cell offsets; ord: row index of every interval.  Row ids must lie in
  * [0, height).  stx_upwind_seg: inst_coef == NULL: one instance (width, a, b,
  * dt, dx, dy as given; dtype 4 rounds them to float); otherwise the batched
- * form of stx_upwind_batched with a table built with row_inst / inst_w. */
+ * form of stx_upwind_batched with a table built with row_inst / inst_w
+ * (instances whose dx is 0 are skipped). */
 int64_t stx_stencil_tile_cells(void);
 int64_t stx_stencil_plan_tiles(int64_t cells);
 int32_t stx_stencil_seg_count(const int32_t* rows, const int32_t* off, int64_t R, const int32_t* beg,
@@ -221,8 +222,9 @@ int32_t stx_value_runs(int32_t esize, int32_t group, const int32_t* off, int64_t
 /* Batched upwind over a stack of independent instances: row k belongs to
  * instance row_inst[k] (width inst_w[i], coefficients inst_coef[5i..5i+4] =
  * a, b, dt, dx, dy in the field type); instance row ids are separated by
- * gaps.  No reference counterpart (the reference advances one instance per
- * call, simulation.py:330-354). */
+ * gaps.  An instance whose dx is 0 is retired: its output cells are left
+ * unwritten.  No reference counterpart (the reference advances one instance
+ * per call, simulation.py:330-354). */
 int32_t stx_upwind_batched(int32_t dtype, int32_t group, const int32_t* rows, const int32_t* off, int64_t R,
                            const int32_t* beg, const int32_t* end, const int32_t* co, const void* u, void* out,
                            const int32_t* row_inst, const int32_t* inst_w, const void* inst_coef, hipStream_t stream);

@@ -194,19 +194,24 @@ class StackedComposite:
         self.cur = 1 - self.cur
 
     def _graph(self, start: int, nsteps: int, stream: torch.cuda.Stream) -> torch.cuda.CUDAGraph:
+        """Graph of ``nsteps`` steps from buffer ``start``, captured on
+        ``stream`` with capture_begin / capture_end (the graph context
+        manager would synchronise the device and empty the allocator cache on
+        every capture, stalling the other streams' hosts)."""
         key = (start, nsteps)
         g = self._graphs.get(key)
         if g is None:
-            stream.wait_stream(torch.cuda.current_stream(self.dev))
             with torch.cuda.stream(stream):
                 context(self.dev)  # per-stream context allocated outside the capture
-            g = torch.cuda.CUDAGraph()
-            with torch.cuda.graph(g, stream=stream):
-                cur = start
-                for _ in range(nsteps):
-                    self._kernels(cur)
-                    cur = 1 - cur
-            torch.cuda.current_stream(self.dev).wait_stream(stream)
+                g = torch.cuda.CUDAGraph()
+                g.capture_begin()
+                try:
+                    cur = start
+                    for _ in range(nsteps):
+                        self._kernels(cur)
+                        cur = 1 - cur
+                finally:
+                    g.capture_end()
             self._graphs[key] = g
         return g
 
@@ -234,6 +239,14 @@ class StackedComposite:
         coarse = self._cbuf[self.cur][c0:c1].clone()
         fine = self._fbuf[self.cur][f0:f1].clone()
         return {"coarse": coarse, "fine": fine, "mass": total_mass_async(coarse, float(self.cell_area[j]))}
+
+    def retire(self, j: int) -> None:
+        """Stop updating instance j (its upwind is skipped from the next
+        launch on; call after its last snapshot).  A device write on the
+        current stream, outside any captured graph, which reads the
+        coefficients at every replay."""
+        for coef in self.inst_coef:
+            coef[5 * j + 3:5 * j + 4].zero_()  # a fill kernel, no host synchronisation
 
     def cells_per_step(self, j: int) -> int:
         """Cell updates of instance j per step (coarse + fine + ghost)."""

@@ -135,7 +135,8 @@ __device__ __forceinline__ T sample_advance(RowCursor<T>& cur, int32_t x, const 
 // BATCH: every row belongs to an instance (row_inst[k]) with its own width
 // inst_w[i] and coefficients inst_coef[5*i .. 5*i+4] = a, b, dt, dx, dy; row
 // ids of different instances are separated by gaps, so the row above / below
-// of a row is found by id equality alone.
+// of a row is found by id equality alone.  An instance with dx == 0 is
+// retired: its rows are skipped (their output cells are not written).
 template <int G, typename T, bool BATCH>
 __global__ __launch_bounds__(kT) void upwind_kernel(const int32_t* __restrict__ rows, const int32_t* __restrict__ off,
                                                     int64_t R, const int32_t* __restrict__ beg,
@@ -157,6 +158,7 @@ __global__ __launch_bounds__(kT) void upwind_kernel(const int32_t* __restrict__ 
         dt = inst_coef[5 * inst + 2];
         dx = inst_coef[5 * inst + 3];
         dy = inst_coef[5 * inst + 4];
+        if (dx == (T)0) return;
     }
     const int32_t o0 = off[k], o1 = off[k + 1];
     if (o0 >= o1) return;

@@ -243,13 +243,6 @@ __device__ __forceinline__ void upwind_group(const SG& sg, int32_t s0, int32_t s
     int32_t s = lo;
     Seg g = sg.get(s);
     if (c + G <= C && c + G <= g.cend) {
-        const VT cv = *(const VT*)(u + c);
-        VT dv = load_dword_aligned(u + (g.dd != kNone ? c + g.dd : c));
-        VT uv = load_dword_aligned(u + (g.ud != kNone ? c + g.ud : c));
-        const T lv = gather<T>(rs, c - 1);
-        const T rv = gather<T>(rs, c + G);
-        if (g.dd == kNone) dv = (VT)(T)0;
-        if (g.ud == kNone) uv = (VT)(T)0;
         if (BATCH) {
             const T* q = inst_coef + 5 * (int64_t)g.inst;
             a = q[0];
@@ -257,7 +250,15 @@ __device__ __forceinline__ void upwind_group(const SG& sg, int32_t s0, int32_t s
             dt = q[2];
             dx = q[3];
             dy = q[4];
+            if (dx == (T)0) return;  // retired instance
         }
+        const VT cv = *(const VT*)(u + c);
+        VT dv = load_dword_aligned(u + (g.dd != kNone ? c + g.dd : c));
+        VT uv = load_dword_aligned(u + (g.ud != kNone ? c + g.ud : c));
+        const T lv = gather<T>(rs, c - 1);
+        const T rv = gather<T>(rs, c + G);
+        if (g.dd == kNone) dv = (VT)(T)0;
+        if (g.ud == kNone) uv = (VT)(T)0;
         const bool inner = g.flags & 2;
         const bool lfirst = c == g.c0 ? (g.flags & 1) : inner;
         const bool rlast = c + G == g.cend ? (g.flags & 4) : inner;
@@ -281,13 +282,6 @@ __device__ __forceinline__ void upwind_group(const SG& sg, int32_t s0, int32_t s
                 ++s;
                 g = sg.get(s);
             }
-            const bool lok = (cc == g.c0) ? (g.flags & 1) : (g.flags & 2);
-            const bool rok = (cc + 1 == g.cend) ? (g.flags & 4) : (g.flags & 2);
-            const T center = u[cc];
-            const T left = lok ? u[cc - 1] : (T)0;
-            const T right = rok ? u[cc + 1] : (T)0;
-            const T down = gather<T>(rs, g.dd != kNone ? cc + g.dd : -1);
-            const T up = gather<T>(rs, g.ud != kNone ? cc + g.ud : -1);
             T ca = a, cb = b, cdt = dt, cdx = dx, cdy = dy;
             if (BATCH) {
                 const T* q = inst_coef + 5 * (int64_t)g.inst;
@@ -296,7 +290,15 @@ __device__ __forceinline__ void upwind_group(const SG& sg, int32_t s0, int32_t s
                 cdt = q[2];
                 cdx = q[3];
                 cdy = q[4];
+                if (cdx == (T)0) continue;  // retired instance
             }
+            const bool lok = (cc == g.c0) ? (g.flags & 1) : (g.flags & 2);
+            const bool rok = (cc + 1 == g.cend) ? (g.flags & 4) : (g.flags & 2);
+            const T center = u[cc];
+            const T left = lok ? u[cc - 1] : (T)0;
+            const T right = rok ? u[cc + 1] : (T)0;
+            const T down = gather<T>(rs, g.dd != kNone ? cc + g.dd : -1);
+            const T up = gather<T>(rs, g.ud != kNone ? cc + g.ud : -1);
             out[cc] = upwind_cell<T>(center, left, right, down, up, ca, cb, cdt, cdx, cdy);
         }
     }

@@ -239,6 +239,7 @@ class Cohort:
                 for j, k in enumerate(self.steps):
                     if k == self.step and j not in self.snaps:
                         self.snaps[j] = self.comp.snapshot(j)
+                        self.comp.retire(j)  # no more work for a finished instance
 
         snap()
         pending = [k for k in self.done_steps if k > self.step]
@@ -301,30 +302,71 @@ def _stack_queries(reqs: List[Request], dev):
     return cat(parts_a), cat(parts_b), ranges
 
 
-def run_queries(reqs: List[Request], dev) -> List[dict]:
-    """All three operations for a batch of queries: one fused launch per
-    operation over the row-stacked pairs (rows of the pairs are identical per
-    query, so no row alignment is needed).  Returns per-query host results."""
+class QueryBatch:
+    """Set-algebra queries answered together on one stream: one fused launch
+    per operation over the row-stacked pairs (rows of the pairs are identical
+    per query, so no row alignment is needed), outputs copied to pinned host
+    buffers without a host synchronisation.  ``collect`` splits them per
+    query once ``ready``."""
+
+    def __init__(self, reqs: List[Request], dev, stream: torch.cuda.Stream):
+        self.reqs = reqs
+        self.stream = stream
+        self.host = {}
+        self._keep = []
+        with torch.cuda.stream(stream):
+            A, B, self.ranges = _stack_queries(reqs, dev)
+            R = A.row_count
+            cap = max(A.interval_count + B.interval_count, 1)  # bounds every operation's output
+            for op, name in ((0, "union"), (1, "intersection"), (2, "difference")):
+                out_off, ob, oe, ctx = _core.setop_raw(op, (A.begin, A.end, A.row_offsets),
+                                                       (B.begin, B.end, B.row_offsets), nrows=R, capacity=cap)
+                h = tuple(torch.empty(t.numel(), dtype=torch.int32, pin_memory=True) for t in (out_off, ob, oe))
+                for dst, src in zip(h, (out_off, ob, oe)):
+                    dst.copy_(src, non_blocking=True)
+                self.host[name] = h
+                self._keep += [out_off, ob, oe]
+            self._keep += [A, B]
+            # status word of the three launches, read without a host wait
+            self.status = torch.empty(1, dtype=torch.int32, pin_memory=True)
+            self.status.copy_(ctx.words[:1], non_blocking=True)
+            self.ctx = ctx
+            self.event = torch.cuda.Event()
+            self.event.record(stream)
+
+    def ready(self) -> bool:
+        return self.event.query()
+
+    def collect(self, keep_outputs: bool = True) -> List[dict]:
+        """Per-query results (empty dicts unless ``keep_outputs``); raises if
+        a launch flagged an error (call when ``ready``)."""
+        if int(self.status[0]):
+            with torch.cuda.stream(self.stream):
+                self.ctx.check_status()  # raises with the flags' meaning
+        if not keep_outputs:
+            self._keep = []
+            return [{} for _ in self.reqs]
+        outs = {name: tuple(t.numpy() for t in h) for name, h in self.host.items()}
+        res = []
+        k = 0
+        for (r0, nr) in self.ranges:
+            one = {}
+            for name, (off, ob, oe) in outs.items():
+                o = off[k:k + nr + 1].copy()
+                one[name] = (o - o[0], ob[o[0]:o[-1]].copy(), oe[o[0]:o[-1]].copy())
+            res.append(one)
+            k += nr + 16
+        self._keep = []
+        return res
+
+
+def run_queries(reqs: List[Request], dev, keep_outputs: bool = True) -> List[dict]:
+    """A batch of queries on the current stream, waiting for the results."""
     if not reqs:
         return []
-    A, B, ranges = _stack_queries(reqs, dev)
-    R = A.row_count
-    outs = {}
-    for op, name in ((0, "union"), (1, "intersection"), (2, "difference")):
-        out_off, ob, oe, ctx = _core.setop_raw(op, (A.begin, A.end, A.row_offsets), (B.begin, B.end, B.row_offsets),
-                                               nrows=R)
-        (total,) = _core.read_counts(ctx, ctx.words[1])
-        outs[name] = (out_off.cpu().numpy(), ob[:total].cpu().numpy(), oe[:total].cpu().numpy())
-    res = []
-    k = 0
-    for (r0, nr) in ranges:
-        one = {}
-        for name, (off, ob, oe) in outs.items():
-            o = off[k:k + nr + 1]
-            one[name] = (o - o[0], ob[o[0]:o[-1]], oe[o[0]:o[-1]])
-        res.append(one)
-        k += nr + 16
-    return res
+    qb = QueryBatch(reqs, dev, torch.cuda.current_stream(dev))
+    qb.event.synchronize()
+    return qb.collect(keep_outputs)
 
 
 # ------------------------------------------------------------------ server
@@ -373,10 +415,12 @@ class Server:
         self.keep_outputs = keep_outputs
         self.window = float(window)
         self.max_cohort = max(1, int(max_cohort))
+        self.qstream = torch.cuda.Stream(self.dev)
 
     def run(self, trace: List[Request]) -> ServeStats:
         pending = deque(sorted(trace, key=lambda r: r.arrival))
         cohorts: List[Cohort] = []
+        qbatches: List[QueryBatch] = []
         waiting: List[Request] = []  # admitted advection jobs without a cohort yet
         wait_since = 0.0
         admitted_at: Dict[int, float] = {}
@@ -391,7 +435,7 @@ class Server:
         t0 = time.perf_counter()
         now = lambda: time.perf_counter() - t0
 
-        while pending or cohorts or waiting or self.sched.waiting:
+        while pending or cohorts or qbatches or waiting or self.sched.waiting:
             t = now()
             while pending and pending[0].arrival <= t:
                 self.sched.offer(pending.popleft())
@@ -405,16 +449,6 @@ class Server:
                     if not waiting:
                         wait_since = now()
                     waiting.append(r)
-            if queries:
-                iters += 1
-                outs = run_queries(queries, self.dev)
-                t_q = now()
-                for r, o in zip(queries, outs):
-                    qints += 3 * (int(r.payload["a"]["off"][-1]) + int(r.payload["b"]["off"][-1]))
-                    results.append(Result(r.rid, "query", r.arrival, admitted_at[r.rid], t_q,
-                                          o if self.keep_outputs else {}))
-                    self.sched.release(r)
-                progressed = True
             if waiting and (not cohorts or len(waiting) >= self.max_cohort or now() - wait_since >= self.window):
                 by_dtype: Dict[torch.dtype, List[Request]] = {}
                 for r in waiting[: self.max_cohort]:
@@ -450,9 +484,28 @@ class Server:
                     still.append(c)
                 progressed = True
             cohorts = still
+            # queries after the cohorts: their host work never delays a cohort
+            if queries:
+                iters += 1
+                self.qstream.wait_stream(main)
+                qbatches.append(QueryBatch(queries, self.dev, self.qstream))
+                progressed = True
+            still_q = []
+            for qb in qbatches:
+                if not qb.ready():
+                    still_q.append(qb)
+                    continue
+                outs = qb.collect(self.keep_outputs)
+                t_q = now()
+                for r, o in zip(qb.reqs, outs):
+                    qints += 3 * (int(r.payload["a"]["off"][-1]) + int(r.payload["b"]["off"][-1]))
+                    results.append(Result(r.rid, "query", r.arrival, admitted_at[r.rid], t_q, o))
+                    self.sched.release(r)
+                progressed = True
+            qbatches = still_q
             if not progressed:
                 nxt = pending[0].arrival - now() if pending else 0.0
-                time.sleep(min(max(nxt, 0.0), 2e-4) if (cohorts or waiting or nxt > 0) else 0.0)
+                time.sleep(min(max(nxt, 0.0), 2e-4) if (cohorts or qbatches or waiting or nxt > 0) else 0.0)
         torch.cuda.synchronize(self.dev)
         wall = now()
         st = ServeStats(sorted(results, key=lambda r: r.rid), wall, cell_updates, qints, iters, self.sched.peak)
@@ -476,7 +529,7 @@ def plain_loop(trace: List[Request], device=None, keep_outputs: bool = False) ->
             time.sleep(min(r.arrival - now(), 0.01))
         ta = now()
         if r.kind == "query":
-            o = run_queries([r], dev)[0]
+            o = run_queries([r], dev, keep_outputs)[0]
             qints += 3 * (int(r.payload["a"]["off"][-1]) + int(r.payload["b"]["off"][-1]))
             results.append(Result(r.rid, "query", r.arrival, ta, now(), o if keep_outputs else {}))
         else:

@@ -1,7 +1,8 @@
 """GPU: stacked two-level composites (the serving cohorts) against each
 instance run alone with CompositeTwoLevel: coarse and extended fine fields
-and FP64 mass bit-exact after each instance's own step count, eager steps
-and captured-graph replays, float32 and float64."""
+and FP64 mass bit-exact after each instance's own step count (instances
+retired once copied out), eager steps and captured-graph replays, float32
+and float64."""
 
 import numpy as np
 import pytest
@@ -71,6 +72,7 @@ def test_stacked_equals_alone(dt, graphs):
                 with torch.cuda.stream(stream) if stream is not None else torch.cuda.stream(
                         torch.cuda.current_stream(dev)):
                     snaps[j] = st.snapshot(j)
+                    st.retire(j)  # later steps skip it; the others must not change
     torch.cuda.synchronize()
     for j, sp in enumerate(specs):
         print("stacked instance", j, dt, "graphs", graphs, flush=True)
