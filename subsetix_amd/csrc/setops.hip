@@ -278,6 +278,27 @@ __device__ __forceinline__ bool key_a_first(int32_t akb, int32_t bkb, int32_t bk
     return akb <= bkb;
 }
 
+// Copies intervals [0, n) of (b, e) into dst as (begin, end) pairs: each
+// thread moves 4 consecutive intervals with one 16-byte load per array (the
+// arrays are only 4-byte aligned) and two 16-byte LDS stores (dst 16-byte
+// aligned); the last partial group element by element.
+template <int TB>
+__device__ __forceinline__ void stage_pairs(const int32_t* __restrict__ b, const int32_t* __restrict__ e, int32_t n,
+                                            int2* dst, int tid) {
+    for (int32_t k = tid * 4; k < n; k += TB * 4) {
+        if (k + 4 <= n) {
+            int4 vb, ve;
+            __builtin_memcpy(&vb, __builtin_assume_aligned(b + k, 4), 16);
+            __builtin_memcpy(&ve, __builtin_assume_aligned(e + k, 4), 16);
+            int4* d = (int4*)(dst + k);
+            d[0] = make_int4(vb.x, ve.x, vb.y, ve.y);
+            d[1] = make_int4(vb.z, ve.z, vb.w, ve.w);
+        } else {
+            for (int32_t q = k; q < n; ++q) dst[q] = make_int2(b[q], e[q]);
+        }
+    }
+}
+
 // MODE: kFused = count, look-back, write in one launch.  Two-pass form:
 // kCount publishes the tile's output count as its look-back state word
 // (aggregate) and stops; prefix_states_kernel turns all aggregates into
@@ -296,9 +317,10 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(tile_waves_p
     constexpr int kPCAP = TB * VT;
     constexpr int kMaxRPT = max_rows_per_tile<TB>();
     typedef typename MaskOf<VT>::type MaskT;
-    __shared__ int2 s_k[kPCAP];  // (begin, end) of the tile's A then B intervals
+    __shared__ __align__(16) int2 s_k[kPCAP];  // (begin, end) of the tile's A then B intervals
     __shared__ int32_t s_aS[kMaxRPT + 1];
     __shared__ int32_t s_bS[kMaxRPT + 1];
+    __shared__ int32_t s_rs[kMaxRPT + 1];  // merged position where each row starts
     // per thread: emissions before its first position, and its emission mask
     // (fallback path: per-row counts, then their exclusive scan)
     __shared__ int32_t s_tbase[TB + 1];
@@ -322,11 +344,14 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(tile_waves_p
         int32_t bs = B.off ? B.off[row0 + r] - b_base : 0;
         s_aS[r] = as;
         s_bS[r] = bs;
+        s_rs[r] = as + bs;
     }
     __syncthreads();
     const int32_t nA = s_aS[nr], nB = s_bS[nr];
     const int32_t P = nA + nB;
-    const bool fits = P <= kPCAP;
+    // B is staged from an even slot so that its pairs are 16-byte aligned
+    const int32_t bOff = (nA + 1) & ~1;
+    const bool fits = bOff + nB <= kPCAP;
 
     int32_t tile_total = 0;
     // per-position results (registers)
@@ -338,10 +363,10 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(tile_waves_p
     if (fits) {
         if (XF) {
             for (int k = tid; k < nA; k += TB) s_k[k] = make_int2(keyb(A, a_base + k), keye(A, a_base + k));
-            for (int k = tid; k < nB; k += TB) s_k[nA + k] = make_int2(keyb(B, b_base + k), keye(B, b_base + k));
+            for (int k = tid; k < nB; k += TB) s_k[bOff + k] = make_int2(keyb(B, b_base + k), keye(B, b_base + k));
         } else {
-            for (int k = tid; k < nA; k += TB) s_k[k] = make_int2(A.beg[a_base + k], A.end[a_base + k]);
-            for (int k = tid; k < nB; k += TB) s_k[nA + k] = make_int2(B.beg[b_base + k], B.end[b_base + k]);
+            stage_pairs<TB>(A.beg + a_base, A.end + a_base, nA, s_k, tid);
+            if (nB > 0) stage_pairs<TB>(B.beg + b_base, B.end + b_base, nB, s_k + bOff, tid);
         }
         __syncthreads();
         const int32_t p0 = tid * VT;
@@ -350,13 +375,13 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(tile_waves_p
             int lo = 0, hi = nr;
             while (hi - lo > 1) {
                 int mid = (lo + hi) >> 1;
-                if (s_aS[mid] + s_bS[mid] <= p0) lo = mid; else hi = mid;
+                if (s_rs[mid] <= p0) lo = mid; else hi = mid;
             }
             int r = lo;
             int32_t aS = s_aS[r], na = s_aS[r + 1] - aS;
-            int32_t bS = nA + s_bS[r], nb = s_bS[r + 1] - s_bS[r];
-            int32_t rs = s_aS[r] + s_bS[r];
-            int32_t rowend = s_aS[r + 1] + s_bS[r + 1];
+            int32_t bS = bOff + s_bS[r], nb = s_bS[r + 1] - s_bS[r];
+            int32_t rs = s_rs[r];
+            int32_t rowend = s_rs[r + 1];
             // co-rank inside the row
             int32_t l = p0 - rs;
             int32_t clo = l - nb > 0 ? l - nb : 0, chi = l < na ? l : na;
@@ -399,10 +424,10 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(tile_waves_p
                             ++r;
                             aS = s_aS[r];
                             na = s_aS[r + 1] - aS;
-                            bS = nA + s_bS[r];
+                            bS = bOff + s_bS[r];
                             nb = s_bS[r + 1] - s_bS[r];
                             rs = rowend;
-                            rowend = s_aS[r + 1] + s_bS[r + 1];
+                            rowend = s_rs[r + 1];
                         } while (p >= rowend);
                         i = 0;
                         j = 0;
@@ -555,7 +580,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(tile_waves_p
         int32_t local;
         if (fits) {
             // emissions before the row's first position q
-            const int32_t q = s_aS[r] + s_bS[r];
+            const int32_t q = s_rs[r];
             if (q >= P) {
                 local = tile_total;
             } else {
@@ -741,13 +766,15 @@ __global__ void add_i32_kernel(int32_t* p, int64_t n, int32_t v) {
 // Host side (C-ABI).
 #include "stx_host.h"
 
-// Rows per tile for the LDS tile kernel: about 3/4 of the tile capacity at
-// the mean row length (the row-bucket schedule of the tile path).
+// Rows per tile for the LDS tile kernel: kTileFill of the tile capacity at
+// the mean row length (the row-bucket schedule of the tile path); a tile
+// whose rows exceed the capacity takes the in-kernel per-row fallback.
+constexpr double kTileFill = 0.9;
 template <int TB, int VT>
 static int32_t setop_tile_rows(int64_t elems, int64_t rows) {
     double avg = rows > 0 ? (double)elems / (double)rows : 1.0;
     if (avg < 1.0) avg = 1.0;
-    int64_t r = (int64_t)(0.75 * TB * VT / avg);
+    int64_t r = (int64_t)(kTileFill * TB * VT / avg);
     if (r < 1) r = 1;
     if (r > max_rows_per_tile<TB>()) r = max_rows_per_tile<TB>();
     return (int32_t)r;
