@@ -532,6 +532,9 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(tile_waves_p
         if (tid == 0) state_store(&lb.state[tile], epoch, 1u, (uint32_t)tile_total);
         return;
     }
+    // the aggregate goes out before the output staging, so that later tiles
+    // find it sooner
+    if (MODE == kFused && tid == 0) lookback_publish(lb, tile, (uint32_t)tile_total, epoch);
     if (fits) {
         // stage the tile's output intervals in LDS (the input stage is free
         // once every walk is done; emissions <= positions <= kPCAP) before
@@ -568,7 +571,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(tile_waves_p
         }
     }
     if (MODE == kFused && wave_id() == 0) {
-        uint32_t e = lookback_exclusive(lb, tile, (uint32_t)tile_total, epoch, status, spin_limit);
+        uint32_t e = lookback_exclusive<1, true>(lb, tile, (uint32_t)tile_total, epoch, status, spin_limit);
         if (lane_id() == 0) s_excl = e;
     }
     __syncthreads();

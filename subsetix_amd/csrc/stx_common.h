@@ -151,15 +151,22 @@ __device__ __forceinline__ unsigned long long state_load(const unsigned long lon
 // 64 * K tiles back in one round trip.  After `spin_limit` polls the block
 // records STX_FLAG_SPIN in `status` and proceeds (its result is then invalid,
 // but every wave still exits).
-template <int K = 1>
+// Publishes tile `tile`'s aggregate ahead of its look-back (one thread):
+// tile 0's aggregate is already its inclusive prefix.
+__device__ __forceinline__ void lookback_publish(LookbackScratch s, uint32_t tile, uint32_t agg, uint32_t epoch) {
+    state_store(&s.state[tile], epoch, tile == 0 ? 2u : 1u, agg);
+}
+
+// PUBLISHED: the aggregate was stored by lookback_publish already.
+template <int K = 1, bool PUBLISHED = false>
 __device__ __forceinline__ uint32_t lookback_exclusive(LookbackScratch s, uint32_t tile, uint32_t agg, uint32_t epoch,
                                                        uint32_t* status, int64_t spin_limit) {
     const int lane = lane_id();
     if (tile == 0) {
-        if (lane == 0) state_store(&s.state[0], epoch, 2u, agg);
+        if (!PUBLISHED && lane == 0) state_store(&s.state[0], epoch, 2u, agg);
         return 0u;
     }
-    if (lane == 0) state_store(&s.state[tile], epoch, 1u, agg);
+    if (!PUBLISHED && lane == 0) state_store(&s.state[tile], epoch, 1u, agg);
     uint32_t excl = 0;
     int64_t pos = (int64_t)tile - 1;  // nearest predecessor not yet folded in
     int64_t spins = 0;
