@@ -337,8 +337,16 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(tile_waves_p
 
     // row0 <= U for every tile (ntiles = floor(U / RPT) + 1), so off[row0]
     // exists; the trailing tile (nr == 0 when RPT divides U) then sees P = 0.
+    // The tile's extent comes straight from the offsets, so that the row
+    // offsets and the intervals are loaded together before one barrier.
     const int32_t a_base = A.off[row0];
     const int32_t b_base = B.off ? B.off[row0] : 0;
+    const int32_t nA = A.off[row0 + nr] - a_base;
+    const int32_t nB = B.off ? B.off[row0 + nr] - b_base : 0;
+    const int32_t P = nA + nB;
+    // B is staged from an even slot so that its pairs are 16-byte aligned
+    const int32_t bOff = (nA + 1) & ~1;
+    const bool fits = bOff + nB <= kPCAP;
     for (int r = tid; r <= nr; r += TB) {
         int32_t as = A.off[row0 + r] - a_base;
         int32_t bs = B.off ? B.off[row0 + r] - b_base : 0;
@@ -346,12 +354,6 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(tile_waves_p
         s_bS[r] = bs;
         s_rs[r] = as + bs;
     }
-    __syncthreads();
-    const int32_t nA = s_aS[nr], nB = s_bS[nr];
-    const int32_t P = nA + nB;
-    // B is staged from an even slot so that its pairs are 16-byte aligned
-    const int32_t bOff = (nA + 1) & ~1;
-    const bool fits = bOff + nB <= kPCAP;
 
     int32_t tile_total = 0;
     // per-position results (registers)
@@ -368,7 +370,9 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(tile_waves_p
             stage_pairs<TB>(A.beg + a_base, A.end + a_base, nA, s_k, tid);
             if (nB > 0) stage_pairs<TB>(B.beg + b_base, B.end + b_base, nB, s_k + bOff, tid);
         }
-        __syncthreads();
+    }
+    __syncthreads();  // row offsets and staged intervals
+    if (fits) {
         const int32_t p0 = tid * VT;
         if (p0 < P) {
             // row of p0
