@@ -154,42 +154,31 @@ def _device_set(h: dict, dev, row_shift: int = 0) -> IntervalSet:
 
 
 class AdvectionJob:
-    def __init__(self, req: Request, dev, stream: Optional[torch.cuda.Stream], use_graph: bool):
+    """One advection job as the plain loop runs it: its own CompositeTwoLevel,
+    eager steps."""
+
+    def __init__(self, req: Request, dev):
         from .amr2.composite import CompositeTwoLevel
         from .multilevel import restrict_set
 
         p = req.payload
         self.req = req
         self.remaining = int(p["steps"])
-        self.stream = stream
-        self.event: Optional[torch.cuda.Event] = None
         discs = _device_set(p["discs"], dev)
         nc = p["n_fine"] // 2
         self.comp = CompositeTwoLevel(nc, nc, restrict_set(discs, 2), dtype=p["dtype"], velocity=p["velocity"])
-        # initial values U[0, 1) from a seeded device generator (same values
-        # whichever stream or server runs the job)
+        # initial values U[0, 1) from a seeded device generator (a cohort
+        # draws its instances' values the same way)
         gen = torch.Generator(device=dev)
         gen.manual_seed(int(p["seed"]))
         cv = torch.rand(nc * nc, generator=gen, device=dev, dtype=torch.float32)
         fv = torch.rand(self.comp.fine_field.values.numel(), generator=gen, device=dev, dtype=torch.float32)
         self.comp.set_values(cv, fv)
-        self.graph = self.comp.capture(stream) if (use_graph and stream is not None and self.remaining >= 2) else None
-        self.graph_parity = self.comp._cur
         self.cells_per_step = self.comp.stats().cell_updates_per_step
 
-    def advance(self, chunk: int) -> int:
-        """Launch up to ``chunk`` steps (no host synchronisation)."""
-        n = min(chunk, self.remaining)
-        done = 0
-        if self.graph is not None and self.comp._cur == self.graph_parity:
-            while n - done >= 2:
-                self.graph.replay()
-                done += 2
-        while done < n:
-            self.comp.step()
-            done += 1
-        self.remaining -= n
-        return n
+    def step(self) -> None:
+        self.comp.step()
+        self.remaining -= 1
 
     def output(self) -> dict:
         return {"mass": self.comp.mass(), "coarse": self.comp.coarse.values, "fine": self.comp.fine_field.values}
@@ -534,10 +523,10 @@ def plain_loop(trace: List[Request], device=None, keep_outputs: bool = False) ->
             results.append(Result(r.rid, "query", r.arrival, ta, now(), o if keep_outputs else {}))
         else:
             ts = time.perf_counter()
-            job = AdvectionJob(r, dev, None, False)
+            job = AdvectionJob(r, dev)
             setup += time.perf_counter() - ts
             while job.remaining:
-                job.advance(1)
+                job.step()
                 torch.cuda.synchronize(dev)
                 cell_updates += job.cells_per_step
             out = job.output()
