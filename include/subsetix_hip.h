@@ -124,6 +124,21 @@ int32_t stx_prolong_set(const int32_t* rows, const int32_t* off, int64_t K, cons
 /* Rows of the prolongation of an empty set (multilevel.py:225-232). */
 int32_t stx_prolong_rows(const int32_t* rows, int64_t K, int32_t ratio, int32_t* frows, hipStream_t stream);
 int32_t stx_fill_i32(int32_t* p, int64_t n, int32_t value, int32_t step, hipStream_t stream);
+/* Box dilation by [-hx, hx] x [-hy, hy] (dilate_interval_set,
+ * morphology.py:211-265: the union of the translated copies), for
+ * 1 <= hy <= stx_dilate_max_halo(): rows holding intervals are compacted,
+ * the output rows are those within hy of one of them, and each output row
+ * is the union of its (<= 2 hy + 1) input rows widened by hx.  Outputs:
+ * rows_out / off_out (row_cap >= R (2 hy + 1), off_out row_cap + 1 entries),
+ * beg_out / end_out (capacity >= (2 hy + 1) N), *d_ny rows, *d_total
+ * intervals.  tmp: stx_dilate_tmp_ints(R, hy) int32.  Three scans on the
+ * scratch use epochs epoch, epoch + 1, epoch + 2. */
+int32_t stx_dilate_max_halo(void);
+int64_t stx_dilate_tmp_ints(int64_t R, int32_t hy);
+int32_t stx_dilate(const int32_t* rows, const int32_t* off, int64_t R, const int32_t* beg, const int32_t* end,
+                   int32_t hx, int32_t hy, int32_t* rows_out, int32_t* off_out, int32_t* beg_out, int32_t* end_out,
+                   int64_t row_cap, int64_t capacity, int32_t* d_ny, int32_t* d_total, int32_t* tmp, int64_t tmp_ints,
+                   uint32_t* status, void* scratch, int64_t scratch_bytes, uint32_t epoch, hipStream_t stream);
 /* translate_interval_set (morphology.py:115-140): dst = src + v. */
 int32_t stx_add_i32(const int32_t* src, int32_t* dst, int64_t n, int32_t v, hipStream_t stream);
 

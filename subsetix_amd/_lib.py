@@ -143,16 +143,18 @@ class Context:
     def slot(self, k: int) -> int:
         return self.words.data_ptr() + 4 * (1 + k)
 
-    def scratch(self, nbytes: int) -> Tuple[int, int, int]:
-        """Look-back scratch of at least ``nbytes`` and a fresh launch epoch
-        (state words of earlier launches then read as unpublished)."""
+    def scratch(self, nbytes: int, count: int = 1) -> Tuple[int, int, int]:
+        """Look-back scratch of at least ``nbytes`` and ``count`` fresh
+        consecutive launch epochs, the first returned (state words of earlier
+        launches then read as unpublished)."""
         if nbytes > self._scratch.numel():
             self._scratch = torch.zeros(max(nbytes, 2 * self._scratch.numel()), dtype=torch.uint8, device=self.device)
-        self._epoch += 1
-        if self._epoch >= (1 << 30) - 1:
+        if self._epoch + count >= (1 << 30) - 1:
             self._scratch.zero_()
-            self._epoch = 1
-        return self._scratch.data_ptr(), self._scratch.numel(), self._epoch
+            self._epoch = 0
+        first = self._epoch + 1
+        self._epoch += count
+        return self._scratch.data_ptr(), self._scratch.numel(), first
 
     def read_slots(self, n: int):
         """One D2H copy: [status, slot0, slot1, ...]; raises on a device flag."""

@@ -343,3 +343,33 @@ def test_stencil_schedules_vs_oracle(sx, schedule):
                 np.testing.assert_array_equal(got64, ox.upwind(s, u.astype(np.float64), **kw))
     finally:
         st.set_stencil_schedule("auto")
+
+
+@pytest.mark.parametrize("hy", [1, 2, 3, 4, 5])
+def test_dilate_windows_vs_oracle(sx, hy):
+    """Dilation on sparse rows with gaps, empty rows kept in the input,
+    negative row ids and touching intervals, hx 0..3, against the oracle:
+    through dilate_interval_set (fused kernel for hy <= 2, row doubling
+    above) and through the fused kernel directly for every hy it takes."""
+    rng = np.random.default_rng(40 + hy)
+    rows = np.sort(rng.choice(np.arange(-30, 200), 90, replace=False))
+    s = random_canonical(rng, 90, 300, mean_n=3, rows=rows, touching=True)
+    # empty rows in the input (zero intervals) must not produce output rows
+    off = s.off.copy()
+    keep = rng.random(s.rows.size) < 0.8
+    beg, end, noff = [], [], [0]
+    for r in range(s.rows.size):
+        if keep[r]:
+            beg += list(s.beg[s.off[r]:s.off[r + 1]])
+            end += list(s.end[s.off[r]:s.off[r + 1]])
+        noff.append(len(beg))
+    s2 = ox.make(noff, beg, end, s.rows)
+    from subsetix_amd import _lib
+    from subsetix_amd.morphology import _dilate_fused
+
+    for hx in (0, 1, 3):
+        print("dilate hy", hy, "hx", hx, flush=True)
+        exp = ox.dilate(s2, hx, hy)
+        assert_set_equal(to_host(sx.dilate_interval_set(to_device(s2), hx, hy)), exp)
+        if hy <= _lib.lib().stx_dilate_max_halo():
+            assert_set_equal(to_host(_dilate_fused(to_device(s2), hx, hy)), exp)
