@@ -50,12 +50,6 @@ from .multilevel import prolong_set, restrict_set
 
 
 
-def _cells(s: IntervalSet) -> int:
-    if s.interval_count == 0:
-        return 0
-    return int(cell_offsets(s)[-1].item())
-
-
 @dataclass
 class BatchConfig:
     levels: int = 3
@@ -312,8 +306,13 @@ class BatchedAMR:
 
     # ------------------------------------------------------------------ stats
     def cells_per_step(self) -> Dict[str, int]:
-        lv = {f"level{i}": _cells(self.level_sets[i]) for i in range(self.cfg.levels)}
-        lv["ghost"] = sum(_cells(g) for g in self.ghosts[1:])
+        """Cells of every level set and of the ghost rings (one host read)."""
+        sets = list(self.level_sets) + list(self.ghosts[1:])
+        ends = [cell_offsets(s)[-1:] for s in sets if s.interval_count]
+        vals = iter(torch.cat(ends).tolist() if ends else [])
+        cells = [int(next(vals)) if s.interval_count else 0 for s in sets]
+        lv = {f"level{i}": cells[i] for i in range(self.cfg.levels)}
+        lv["ghost"] = sum(cells[self.cfg.levels:])
         return lv
 
     def masses(self) -> torch.Tensor:

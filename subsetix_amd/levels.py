@@ -103,10 +103,13 @@ def _seg_possible(src: IntervalField, ratio: int) -> bool:
     return ratio == 2 and src.values.numel() * src.values.element_size() < (1 << 32) - 256
 
 
-def _transfer(mode: int, target: IntervalSet, dst: IntervalField, src: IntervalField, ratio: int, acc64: int) -> None:
+def _transfer(mode: int, target: IntervalSet, dst: IntervalField, src: IntervalField, ratio: int, acc64: int,
+              schedule=None) -> None:
     """One fused transfer with the schedule chosen as for the stencil:
-    forced, else the autotuner's (candidates run on a copy of the
-    destination), else segment tables when they apply."""
+    ``schedule`` ("rows" for a one-shot transfer, where building a segment
+    table costs more than it saves), else the module's forced schedule, else
+    the autotuner's (candidates run on a copy of the destination), else
+    segment tables when they apply."""
     ctx = context(dst.values.device)
     dcode = _dtype_code(dst.values)
     seg_ok = _seg_possible(src, ratio)
@@ -125,9 +128,10 @@ def _transfer(mode: int, target: IntervalSet, dst: IntervalField, src: IntervalF
                                       *_geom(src.interval_set, src.interval_cell_offsets), ptr(src.values),
                                       ptr(d.values), ctx.stream), "prolong_into" if mode == 0 else "restrict_into")
 
-    fixed = "seg" if (seg_ok and _TRANSFER_SCHEDULE != "rows") else _group(group_cells, target.row_count)
+    fixed = "seg" if (seg_ok and _TRANSFER_SCHEDULE != "rows" and schedule != "rows") else _group(
+        group_cells, target.row_count)
     choice = fixed
-    if _TRANSFER_SCHEDULE == "auto":
+    if _TRANSFER_SCHEDULE == "auto" and schedule is None:
         def factory():
             scratch = IntervalField(dst.interval_set, dst.values.clone(), dst.interval_cell_offsets)
             cands = (("seg",) if seg_ok else ()) + autotune.GROUPS
@@ -140,14 +144,16 @@ def _transfer(mode: int, target: IntervalSet, dst: IntervalField, src: IntervalF
     launch(choice, dst)
 
 
-def prolong_into(target: IntervalSet, dst: IntervalField, src: IntervalField, ratio: int) -> None:
+def prolong_into(target: IntervalSet, dst: IntervalField, src: IntervalField, ratio: int, *,
+                 schedule=None) -> None:
     """dst(y, x) = src(y // ratio, x // ratio) for every cell of ``target``
-    (cells of target must lie in dst's geometry)."""
+    (cells of target must lie in dst's geometry).  ``schedule="rows"``: no
+    segment table (a transfer made once)."""
     if target.interval_count == 0:
         return
     if dst.values.dtype != src.values.dtype:
         raise TypeError("source and destination fields must share a dtype")
-    _transfer(0, target, dst, src, int(ratio), 0)
+    _transfer(0, target, dst, src, int(ratio), 0, schedule)
 
 
 def restrict_into(target: IntervalSet, dst: IntervalField, src: IntervalField, ratio: int, *,
