@@ -24,9 +24,8 @@
 // The update kernel cuts the cells into tiles of kTileCells; a thread owns 8
 // consecutive cells (one segment lookup, vector loads and stores), the
 // block stages the tile's segment records in LDS.  Tiles are mapped to
-// blocks XCD-major (block b runs on XCD b % 8; XCD x gets the tiles
-// [x * per, (x + 1) * per)) so the rows above and below a tile are read
-// through the same L2.
+// blocks in XCD chunks (xcd_chunk_order) so the rows above and below a tile
+// are mostly read through the same L2.
 //
 // Precondition (checked by the Python layer unless disabled): row ids lie in
 // [0, height), so adjacency of row ids decides the vertical neighbours.
@@ -44,6 +43,19 @@ constexpr int kCellsPerThread = 8;
 constexpr int kTileCells = kThreads * kCellsPerThread;
 constexpr int kSegCap = 512;
 constexpr int kXcds = 8;
+// Block -> tile order: the XCDs take chunks of kOrderChunk consecutive tiles
+// in turn (block b runs on XCD b % 8), so the rows above and below a tile are
+// mostly read through the same L2 while the whole chip sweeps the field front
+// to back (measured: same FP32 step time, 10 % faster FP64 steps than one
+// contiguous eighth per XCD).  Grids are padded to a multiple of
+// kOrderSpan, which makes the order a permutation of the padded grid.
+constexpr int64_t kOrderChunk = 8;
+constexpr int64_t kOrderSpan = kOrderChunk * kXcds;
+__device__ __forceinline__ int64_t xcd_chunk_order(int64_t block) {
+    const int64_t xcd = block % kXcds, nth = block / kXcds;
+    return ((nth / kOrderChunk) * kXcds + xcd) * kOrderChunk + nth % kOrderChunk;
+}
+
 constexpr int32_t kNone = INT32_MIN;
 constexpr int kRec = 8;
 
@@ -324,8 +336,7 @@ __global__ __launch_bounds__(kThreads) void upwind_seg_kernel(const int32_t* __r
                                                               T b, T dt, T dx, T dy,
                                                               const T* __restrict__ inst_coef) {
     __shared__ int4 s_seg[2 * kSegCap];
-    const int64_t per = (int64_t)gridDim.x / kXcds;
-    const int64_t t = (int64_t)(blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+    const int64_t t = xcd_chunk_order(blockIdx.x);
     if (t >= ntiles) return;
     const int32_t s0 = plan[2 * t], s1 = plan[2 * t + 1];
     const int32_t M = s1 - s0;
@@ -444,8 +455,7 @@ __global__ __launch_bounds__(kThreads) void tag_bits_kernel(const int32_t* __res
     constexpr int G = Vec<T>::n;
     constexpr int LPW = 32 / G;  // lanes per word
     __shared__ int4 s_seg[2 * kSegCap];
-    const int64_t per = (int64_t)gridDim.x / kXcds;
-    const int64_t t = (int64_t)(blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+    const int64_t t = xcd_chunk_order(blockIdx.x);
     if (t >= ntiles) return;
     const int32_t s0 = plan[2 * t], s1 = plan[2 * t + 1];
     const int32_t M = s1 - s0;
@@ -562,7 +572,7 @@ template <typename T, bool BATCH>
 int32_t launch_upwind_seg(const int32_t* plan, int64_t ntiles, int64_t C, const int32_t* segs, const void* u,
                           void* out, double a, double b, double dt, double dx, double dy, const void* inst_coef,
                           hipStream_t stream) {
-    const int64_t grid = (ntiles + kXcds - 1) / kXcds * kXcds;
+    const int64_t grid = (ntiles + kOrderSpan - 1) / kOrderSpan * kOrderSpan;
     if (grid > 0x7fffffffLL) return stx_fail(STX_EINVAL, "upwind_seg: too many tiles");
     if ((uintptr_t)u % 16 != 0 || (uintptr_t)out % 16 != 0)
         return stx_fail(STX_EINVAL, "upwind_seg: values and output must be 16-byte aligned");
@@ -662,7 +672,7 @@ int32_t stx_grad_tag_seg(int32_t dtype, const int32_t* plan, int64_t ntiles, int
     int32_t* cnt = (int32_t*)(words + 4 * W);
     int32_t* pref = (int32_t*)(words + 5 * W);
     if (W > 0) {
-        const int64_t grid = (ntiles + kXcds - 1) / kXcds * kXcds;
+        const int64_t grid = (ntiles + kOrderSpan - 1) / kOrderSpan * kOrderSpan;
         if (dtype == 4)
             hipLaunchKernelGGL(tag_bits_kernel<float>, dim3((unsigned)grid), dim3(kThreads), 0, stream, plan, ntiles,
                                (int32_t)C, segs, (const float*)u, (float)t, tag_w, lex_w, W, status);
