@@ -77,7 +77,7 @@ def _host_set(h) -> dict:
 
 def advect_request(rid: int, arrival: float, *, n_fine: int, steps: int, seed: int,
                    velocity=(1.0, 0.5), dtype=torch.float32) -> Request:
-    """A config-3-style instance: fine n_fine^2 domain covered ~30% by seeded
+    """A disc-refined two-level instance: fine n_fine^2 domain covered ~30% by seeded
     discs + annulus, coarse (n_fine/2)^2 full level, ratio 2, initial values
     U[0, 1] (seeded), ``steps`` upwind + transfer steps."""
     discs, scale, cov = workloads.stencil_discs_config(n=n_fine, seed=seed)

@@ -72,9 +72,9 @@ def test_setalgebra_config_morphology_full_size(sx, r):
     ha, _ = workloads.setalgebra_pair()
     a = hostset_to_oset(ha)
     da = to_device(a)
-    print("config2 dilate r", r, flush=True)
+    print("4096x65536 set dilate r", r, flush=True)
     assert_set_equal(to_host(m.dilate_interval_set(da, r, r)), ox.dilate(a, r, r))
-    print("config2 erode r", r, flush=True)
+    print("4096x65536 set erode r", r, flush=True)
     assert_set_equal(to_host(m.erode_interval_set(da, r, r, width=65536, height=4096)),
                      ox.erode(a, r, r, width=65536, height=4096))
 

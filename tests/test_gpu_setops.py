@@ -104,7 +104,7 @@ def test_setalgebra_config_full_size(sx):
     a, b = hostset_to_oset(ha), hostset_to_oset(hb)
     da, db = to_device(a), to_device(b)
     for op in ("union", "intersection", "difference"):
-        print("config2", op, flush=True)
+        print("4096x65536 pair", op, flush=True)
         assert_set_equal(to_host(_ops(sx)[op](da, db)), ox.setop(op, a, b))
 
 
