@@ -216,11 +216,17 @@ class StackedComposite:
         return g
 
     def advance(self, n: int, stream: Optional[torch.cuda.Stream] = None, use_graph: bool = True) -> None:
-        """``n`` steps; with a stream and graphs, by replays of captured
-        two-step and one-step graphs (no host synchronisation)."""
-        if not use_graph or stream is None:
+        """``n`` steps on ``stream`` (the current stream when None): eager
+        launches, or with ``use_graph`` replays of captured two-step and
+        one-step graphs (no host synchronisation either way)."""
+        if stream is None:
             for _ in range(n):
                 self.step()
+            return
+        if not use_graph:
+            with torch.cuda.stream(stream):
+                for _ in range(n):
+                    self.step()
             return
         with torch.cuda.stream(stream):
             while n >= 2:

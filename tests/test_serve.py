@@ -73,6 +73,10 @@ def test_results_alone_equal_in_mix():
     plain = plain_loop(trace, keep_outputs=True).results
     for a, b in zip(mix, plain):
         _same(a, b)
+    # eager cohort steps (no graphs) on the cohort streams
+    eager = Server(budget_bytes=budget, streams=2, chunk=3, use_graph=False, keep_outputs=True).run(trace).results
+    for a, b in zip(eager, plain):
+        _same(a, b)
     # query results against the oracle
     for r in trace:
         if r.kind != "query":
