@@ -225,6 +225,86 @@ __device__ __forceinline__ uint32_t lookback_exclusive(LookbackScratch s, uint32
 // Bytes of look-back scratch needed for `ntiles` tiles.
 __host__ __device__ inline int64_t lookback_bytes(int64_t ntiles) { return 64 + 8 * ntiles; }
 
+// ---- wave scans through DPP ------------------------------------------------
+// Hillis-Steele inside each 16-lane row (row_shr 1, 2, 4, 8), then
+// row_bcast:15 / row_bcast:31 carry the row totals across the wave (GFX9
+// DPP forms, present on CDNA).  Lanes without a source take `id`, the
+// identity of the operation.
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ int32_t dpp_mov(int32_t id, int32_t v) {
+    return __builtin_amdgcn_update_dpp(id, v, CTRL, ROW_MASK, 0xf, false);
+}
+
+__device__ __forceinline__ int32_t wave_incl_sum(int32_t x) {
+    x += dpp_mov<0x111>(0, x);
+    x += dpp_mov<0x112>(0, x);
+    x += dpp_mov<0x114>(0, x);
+    x += dpp_mov<0x118>(0, x);
+    x += dpp_mov<0x142, 0xa>(0, x);
+    x += dpp_mov<0x143, 0xc>(0, x);
+    return x;
+}
+
+__device__ __forceinline__ int32_t imax(int32_t a, int32_t b) { return a > b ? a : b; }
+__device__ __forceinline__ int32_t imin(int32_t a, int32_t b) { return a < b ? a : b; }
+
+__device__ __forceinline__ int32_t wave_incl_max(int32_t x) {
+    x = imax(x, dpp_mov<0x111>(INT32_MIN, x));
+    x = imax(x, dpp_mov<0x112>(INT32_MIN, x));
+    x = imax(x, dpp_mov<0x114>(INT32_MIN, x));
+    x = imax(x, dpp_mov<0x118>(INT32_MIN, x));
+    x = imax(x, dpp_mov<0x142, 0xa>(INT32_MIN, x));
+    x = imax(x, dpp_mov<0x143, 0xc>(INT32_MIN, x));
+    return x;
+}
+
+__device__ __forceinline__ int32_t wave_incl_min(int32_t x) {
+    x = imin(x, dpp_mov<0x111>(INT32_MAX, x));
+    x = imin(x, dpp_mov<0x112>(INT32_MAX, x));
+    x = imin(x, dpp_mov<0x114>(INT32_MAX, x));
+    x = imin(x, dpp_mov<0x118>(INT32_MAX, x));
+    x = imin(x, dpp_mov<0x142, 0xa>(INT32_MAX, x));
+    x = imin(x, dpp_mov<0x143, 0xc>(INT32_MAX, x));
+    return x;
+}
+
+// Value of lane 63 (the wave total after an inclusive scan).
+__device__ __forceinline__ int32_t wave_last(int32_t x) { return __builtin_amdgcn_readlane(x, 63); }
+
+// Block-wide exclusive sum over one value per thread with one barrier:
+// `lds` holds one slot per wave and is used once per kernel.
+template <int NW>
+__device__ __forceinline__ int32_t block_excl_sum(int32_t v, int32_t* lds, int32_t* total) {
+    const int wave = wave_id();
+    const int32_t incl = wave_incl_sum(v);
+    if (lane_id() == 63) lds[wave] = incl;
+    __syncthreads();
+    int32_t before = 0, all = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+        const int32_t t = lds[w];
+        before += w < wave ? t : 0;
+        all += t;
+    }
+    *total = all;
+    return before + incl - v;
+}
+
+// Block-wide exclusive max (INT32_MIN before thread 0), one barrier; `lds`
+// holds one slot per wave and is used once per kernel.
+template <int NW>
+__device__ __forceinline__ int32_t block_excl_max(int32_t v, int32_t* lds) {
+    const int wave = wave_id();
+    const int32_t incl = wave_incl_max(v);
+    if (lane_id() == 63) lds[wave] = incl;
+    __syncthreads();
+    int32_t before = INT32_MIN;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) before = w < wave ? imax(before, lds[w]) : before;
+    // wave_shr:1 - the inclusive value of the lane below (lane 0: identity)
+    return imax(before, dpp_mov<0x138>(INT32_MIN, incl));
+}
+
 // Block-wide exclusive scan of one value per thread (blockDim.x <= 1024).
 // Returns the exclusive prefix; *total receives the block sum.
 template <typename T>

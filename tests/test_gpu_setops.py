@@ -55,7 +55,7 @@ def test_golden_cell_model(sx, golden):
             assert ox.cells(to_host(got)) == cell_model(spec, cells), spec
 
 
-@pytest.mark.parametrize("group", [None, -2, -3, -4, 4, 16, 64])  # None / negative: LDS tile shapes
+@pytest.mark.parametrize("group", [None, -2, -3, -4, -5, 4, 16, 64])  # None / negative: LDS tile shapes
 @pytest.mark.parametrize("seed", [0, 1, 2])
 def test_random_vs_oracle(sx, seed, group):
     from subsetix_amd import _core
@@ -197,13 +197,13 @@ def test_row_count_multiple_of_tile_rows(sx):
             assert_set_equal(to_host(_ops(sx)[op](da, db)), ox.setop(op, a, b))
 
 
-@pytest.mark.parametrize("schedule", [-1, -2, -3, -4])
+@pytest.mark.parametrize("schedule", [-1, -2, -3, -4, -5])
 def test_tile_schedules_batched_pairs(sx, schedule):
     """Tile schedules on 16 stacked full-size pairs (hundreds of tiles),
     bit-exact against the oracle: single pass 512 x 8 (-1, decoupled
     look-back), two passes (-2, count kernel, prefix kernel, write kernel),
-    single pass 128 x 16 (-3) and 64 x 32 (-4); repeated launches on one
-    scratch buffer alternate schedules.  Dilation (transforms on load, one
+    single pass 128 x 16 (-3) and 64 x 32 (-4), flat tiles 512 x 8 (-5);
+    repeated launches on one scratch buffer alternate schedules.  Dilation (transforms on load, one
     operand) runs through the same shape."""
     import torch
 
@@ -231,5 +231,100 @@ def test_tile_schedules_batched_pairs(sx, schedule):
         for r in (1, 3):
             print("schedule", schedule, "dilate", r, flush=True)
             assert_set_equal(to_host(sx.dilate_interval_set(to_device(small_a), r, r)), ox.dilate(small_a, r, r))
+    finally:
+        _core.set_group_override(None)
+
+
+def _row_runs(rng, base, n, touching):
+    out, x = [], base + int(rng.integers(0, 4))
+    for _ in range(n):
+        gap = int(rng.geometric(0.3)) - (1 if touching and rng.random() < 0.3 else 0)
+        b = x + max(gap, 0 if touching else 1)
+        e = b + int(rng.geometric(0.25))
+        out.append((b, e))
+        x = e
+    return out
+
+
+def _rows_to_oset(rows):
+    off, beg, end = [0], [], []
+    for lst in rows:
+        for b, e in lst:
+            beg.append(b)
+            end.append(e)
+        off.append(len(beg))
+    return ox.make(off, beg, end)
+
+
+def _flat_edge_pair(seed):
+    """Identity-row pair (2400 rows) whose tiles reach every branch of the
+    flat tile kernel: flat tiles with negative and with large coordinates,
+    tiles whose row bands overflow 31 bits (coordinates near +-1e9 and near
+    the int32 limits: per-row fallback), a row longer than the LDS capacity
+    (per-row fallback), runs of empty rows, rows present in one operand only,
+    touching runs."""
+    rng = np.random.default_rng(seed)
+    A, B = [], []
+    for r in range(2400):
+        base = 0
+        if 200 <= r < 260:
+            base = -5000
+        elif 700 <= r < 704:
+            base = 1_000_000_000 + 40_000_000 * (r - 700)
+        elif 1500 <= r < 1502:
+            base = -2_000_000_000
+        elif 1800 <= r < 1803:
+            base = 3_000_000
+        la = _row_runs(rng, base, int(rng.poisson(12)), r % 7 == 0)
+        lb = _row_runs(rng, base + int(rng.integers(-20, 20)), int(rng.poisson(12)), r % 5 == 0)
+        if 300 <= r < 360:
+            la, lb = [], []
+        elif 400 <= r < 420:
+            lb = []
+        elif 420 <= r < 440:
+            la = []
+        elif r == 1000:
+            la = _row_runs(rng, 0, 5000, False)
+        elif r == 2000:
+            la = [(-2**31 + 1, -2**31 + 9), (2**31 - 20, 2**31 - 2)]
+        elif r == 2001:
+            lb = [(-2**31 + 5, -2**31 + 6), (2**31 - 19, 2**31 - 2)]
+        A.append(la)
+        B.append(lb)
+    return _rows_to_oset(A), _rows_to_oset(B)
+
+
+@pytest.mark.parametrize("schedule", [-1, -5])
+@pytest.mark.parametrize("seed", [0, 1])
+def test_flat_tile_edge_cases(sx, schedule, seed):
+    """Every branch of the flat tile kernel (and the same inputs through the
+    row-aware tile kernel), bit-exact against the oracle; the widen and clip
+    transforms on load run through the same schedule."""
+    from subsetix_amd import _core
+
+    a, b = _flat_edge_pair(seed)
+    da, db = to_device(a), to_device(b)
+    R = da.row_count
+    for op, name in ((0, "union"), (1, "intersection"), (2, "difference")):
+        print("flat edges seed", seed, "schedule", schedule, name, flush=True)
+        exp = ox.setop(name, a, b)
+        out_off, ob, oe, ctx = _core.setop_raw(op, (da.begin, da.end, da.row_offsets),
+                                               (db.begin, db.end, db.row_offsets), nrows=R, group=schedule)
+        (total,) = _core.read_counts(ctx, ctx.words[1])
+        np.testing.assert_array_equal(out_off.cpu().numpy(), exp.off)
+        np.testing.assert_array_equal(ob[:total].cpu().numpy(), exp.beg)
+        np.testing.assert_array_equal(oe[:total].cpu().numpy(), exp.end)
+    # single operand with transforms on load (widen: merge of overlaps; clip:
+    # dead intervals), on the rows whose coordinates keep the results in int32
+    rows = [list(zip(a.beg[a.off[r]:a.off[r + 1]], a.end[a.off[r]:a.off[r + 1]])) for r in range(R)]
+    small = _rows_to_oset([lst if r < 690 else [] for r, lst in enumerate(rows)])
+    ds = to_device(small)
+    _core.set_group_override(schedule)
+    try:
+        for hx in (1, 3):
+            print("flat edges widen", hx, flush=True)
+            assert_set_equal(to_host(sx.dilate_interval_set(ds, hx, 0)), ox.dilate(small, hx, 0))
+        print("flat edges clip", flush=True)
+        assert_set_equal(to_host(sx.clip_interval_set(ds, width=60, height=R)), ox.clip(small, 60, R))
     finally:
         _core.set_group_override(None)
