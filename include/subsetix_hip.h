@@ -72,12 +72,12 @@ int64_t stx_scan_scratch_bytes(int64_t n_bound);
  * _align_interval_sets, expressions.py:58-141) and an optional transform
  * (STX_XF_*).  Output: out_off[U+1], out_beg/out_end (capacity entries),
  * *d_total.  U = *d_nrows when non-NULL, else nrows_bound.
- * group <= 0: LDS tile path when no maps are given, 512 threads x 8
- * positions per tile unless stated: 0 and -5: flat tiles (a tile's rows
- * merged as one keyed sequence, single pass); -1: row-aware tiles, single
- * pass; -2: row-aware tiles in two passes; -3: row-aware 128 x 16; -4:
- * row-aware 64 x 32.  With maps, the per-row path.  4 / 16 / 64: lane-group
- * size of the per-row path. */
+ * group <= 0: LDS tile path when no maps are given: 0 and -6: flat tiles
+ * (a tile's rows merged as one keyed sequence, single pass), 1024 threads x
+ * 8 positions; -5: flat tiles, 512 x 8; -1: row-aware tiles, 512 x 8, single
+ * pass; -2: the same in two passes; -3: row-aware 128 x 16; -4: row-aware
+ * 64 x 32.  With maps, the per-row path.  4 / 16 / 64: lane-group size of
+ * the per-row path. */
 int32_t stx_setop(int32_t op, const int32_t* a_beg, const int32_t* a_end, const int32_t* a_off, const int32_t* a_map,
                   int32_t a_xf_mode, int32_t a_xf_param, int64_t a_n, const int32_t* b_beg, const int32_t* b_end,
                   const int32_t* b_off, const int32_t* b_map, int32_t b_xf_mode, int32_t b_xf_param, int64_t b_n,

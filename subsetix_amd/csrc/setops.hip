@@ -645,6 +645,13 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(tile_waves_p
 // the row of each output and of its predecessor.
 // A tile that does not fit (intervals beyond the LDS capacity, or
 // nr << s >= 2^31) is processed one wavefront per row instead.
+// Rows per flat tile: at most one per thread (the per-row fallback scans one
+// row count per thread).
+template <int TB>
+constexpr int flat_rows_per_tile() {
+    return TB;
+}
+
 template <bool XF, int VT>
 __device__ __forceinline__ void flat_load(const Operand& A, const Operand& B, int32_t a_base, int32_t b_base,
                                           int32_t nA, int32_t nB, int32_t q0, int32_t* xb, int32_t* xe) {
@@ -692,7 +699,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(tile_waves_p
     uint32_t* status, void* scratch, uint32_t epoch, int64_t spin_limit) {
     constexpr int kTW = TB / STX_WAVE;
     constexpr int kPCAP = TB * VT;
-    constexpr int kMaxRPT = max_rows_per_tile<TB>();
+    constexpr int kMaxRPT = flat_rows_per_tile<TB>();
     static_assert(VT >= 4 && VT % 4 == 0 && VT < 32, "VT: multiple of 4, below 32");
     constexpr int32_t a0 = 2;  // first A slot; slot 1 holds A's lower sentinel
     // upper sentinels: x above every key; y below every key (the union's
@@ -1130,20 +1137,22 @@ __global__ void add_i32_kernel(int32_t* p, int64_t n, int32_t v) {
 // whose rows exceed the capacity takes the in-kernel per-row fallback.
 constexpr double kTileFill = 0.9;
 template <int TB, int VT>
-static int32_t setop_tile_rows(int64_t elems, int64_t rows) {
+static int32_t setop_tile_rows(int64_t elems, int64_t rows, double fill = kTileFill, int max_rows = max_rows_per_tile<TB>()) {
     double avg = rows > 0 ? (double)elems / (double)rows : 1.0;
     if (avg < 1.0) avg = 1.0;
-    int64_t r = (int64_t)(kTileFill * TB * VT / avg);
+    int64_t r = (int64_t)(fill * TB * VT / avg);
     if (r < 1) r = 1;
-    if (r > max_rows_per_tile<TB>()) r = max_rows_per_tile<TB>();
+    if (r > max_rows) r = max_rows;
     return (int32_t)r;
 }
 
 constexpr uint32_t kTwoPassTiles = 0xffffffffu;  // measured slower than one pass at every size tried
-// Group 0 takes the flat tiles: at 128 stacked 4096 x 65536 pairs they beat
-// the row-aware tiles by 1-3 % (union 196.6 vs 202.4 us), and tie on one and
-// on 16 pairs.
+// Group 0 takes the 1024 x 8 flat tiles filled to kFlatFill: at 128 stacked
+// 4096 x 65536 pairs, union 178.7 us against 197.7 (512 x 8 flat tiles,
+// filled to 90 %) and 201.0 (row-aware tiles); intersection 165.1 / 175.1 /
+// 176.9; difference 170.3 / 185.4 / 188.5.  All tie on one and on 16 pairs.
 constexpr bool kFlatDefault = true;
+constexpr double kFlatFill = 0.95;
 
 template <int OP, int MODE, int TB, int VT>
 static void launch_tile_op(uint32_t ntiles, hipStream_t stream, const Operand& A, const Operand& B, int64_t U, int32_t rpt,
@@ -1254,13 +1263,23 @@ int32_t stx_setop(int32_t op, const int32_t* a_beg, const int32_t* a_end, const 
     if (tile_path) {
         const int64_t elems = a_n + (b_off ? b_n : 0);
         const int64_t spin = stx_spin_limit();
-        if (group == -5 || (group == 0 && kFlatDefault)) {
+        if (group == -5) {
             // flat tiles (rows merged as one keyed sequence), 512 x 8
-            const int32_t rpt = setop_tile_rows<512, 8>(elems, nrows_bound);
+            const int32_t rpt = setop_tile_rows<512, 8>(elems, nrows_bound, kTileFill, flat_rows_per_tile<512>());
             const uint32_t ntiles = (uint32_t)((nrows_bound + rpt) / rpt);
             if (lookback_bytes(ntiles) > scratch_bytes) return stx_fail(STX_EINVAL, "setop: scratch too small");
             launch_flat<512, 8>(op, ntiles, stream, A, B, nrows_bound, rpt, out_off, out_beg, out_end, capacity, d_total,
                                 status, scratch, epoch, spin);
+            return stx_launch_status("setop_flat_kernel");
+        }
+        if (group == -6 || (group == 0 && kFlatDefault)) {
+            // flat tiles, 1024 x 8 (two per CU), filled to 95 %: the mean
+            // tile holds ~100 rows or more, so its size varies little
+            const int32_t rpt = setop_tile_rows<1024, 8>(elems, nrows_bound, kFlatFill, flat_rows_per_tile<1024>());
+            const uint32_t ntiles = (uint32_t)((nrows_bound + rpt) / rpt);
+            if (lookback_bytes(ntiles) > scratch_bytes) return stx_fail(STX_EINVAL, "setop: scratch too small");
+            launch_flat<1024, 8>(op, ntiles, stream, A, B, nrows_bound, rpt, out_off, out_beg, out_end, capacity,
+                                 d_total, status, scratch, epoch, spin);
             return stx_launch_status("setop_flat_kernel");
         }
         if (group == -3 || group == -4) {
