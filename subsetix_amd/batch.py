@@ -42,7 +42,7 @@ import torch
 from . import _core, _lib, autotune
 from ._lib import _I32, _I64, _P, check, context, ptr
 from .expressions import IntervalSet, binary_op
-from .interval_field import IntervalField, cell_offsets, create_interval_field
+from .interval_field import IntervalField, cell_offsets, create_interval_field, known_cell_count, remember_cell_count
 from .interval_stencil import build_stencil_plan
 from .levels import gradient_tag, prolong_into, restrict_into, transfer_plan, upwind_batched
 from .morphology import dilate_interval_set
@@ -308,9 +308,12 @@ class BatchedAMR:
     def cells_per_step(self) -> Dict[str, int]:
         """Cells of every level set and of the ghost rings (one host read)."""
         sets = list(self.level_sets) + list(self.ghosts[1:])
-        ends = [cell_offsets(s)[-1:] for s in sets if s.interval_count]
+        known = [known_cell_count(s, cell_offsets(s)) for s in sets]
+        ends = [cell_offsets(s)[-1:] for s, k in zip(sets, known) if k is None]
         vals = iter(torch.cat(ends).tolist() if ends else [])
-        cells = [int(next(vals)) if s.interval_count else 0 for s in sets]
+        cells = [int(next(vals)) if k is None else k for k in known]
+        for s, c in zip(sets, cells):
+            remember_cell_count(s, cell_offsets(s), c)
         lv = {f"level{i}": cells[i] for i in range(self.cfg.levels)}
         lv["ghost"] = sum(cells[self.cfg.levels:])
         return lv

@@ -14,7 +14,7 @@ import torch
 from . import _core, _lib, autotune
 from ._lib import _F64, _I32, _I64, _P, check, context, ptr
 from .expressions import IntervalSet
-from .interval_field import IntervalField, cell_offsets
+from .interval_field import IntervalField, cell_offsets, known_cell_count, remember_cell_count
 
 _lib.register("stx_transfer", _I32, [_I32, _I32, _I32, _I32, _I32, _P, _P, _P, _P, _P, _I64, _P, _P, _P, _P, _P, _I64,
                                      _P, _P, _P, _P, _P, _I64, _P, _P, _P])
@@ -63,9 +63,7 @@ def transfer_plan(mode: int, target: IntervalSet, dst: IntervalField, src: Inter
     ctx = context(dev)
     tco = cell_offsets(target)
     n = target.interval_count
-    cells = int(tco[-1].item()) if n else 0
-    tile = int(_lib.lib().stx_transfer_tile_cells())
-    ntiles = (cells + tile - 1) // tile
+    known = known_cell_count(target, tco)
     ord_ = target.interval_row_ordinals() if n else _core.empty_i32(1, dev)
     counts = _core.empty_i32(max(n, 1), dev)
     seg_off = _core.empty_i32(n + 1, dev)
@@ -76,7 +74,13 @@ def transfer_plan(mode: int, target: IntervalSet, dst: IntervalField, src: Inter
     check(_lib.lib().stx_transfer_seg_count(mode, *tg, ptr(ord_), n, *dg, *sgm, ptr(counts), ptr(seg_off),
                                             ctx.slot(0), ctx.status_ptr, sp, sbytes, ep, ctx.stream),
           "transfer_seg_count")
-    (nseg,) = _core.read_counts(ctx, ctx.words[1])
+    if known is None:  # the cell count rides on the segment count's read
+        nseg, cells = _core.read_counts(ctx, ctx.words[1], tco[-1])
+        remember_cell_count(target, tco, cells)
+    else:
+        (nseg,), cells = _core.read_counts(ctx, ctx.words[1]), known
+    tile = int(_lib.lib().stx_transfer_tile_cells())
+    ntiles = (cells + tile - 1) // tile
     segs = _core.empty_i32(8 * max(nseg, 1), dev)
     plan = _core.empty_i32(2 * max(ntiles, 1), dev)
     check(_lib.lib().stx_transfer_seg_build(mode, *tg, ptr(ord_), n, *dg, *sgm, ptr(seg_off), nseg, ptr(segs), cells,
