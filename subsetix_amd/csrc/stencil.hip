@@ -21,9 +21,11 @@
 // neighbour (cell c_end).  dd / ud = kNone when there is no vertical
 // neighbour (row absent, column not held, or segment outside [0, w)).
 //
-// The update kernel cuts the cells into tiles of kTileCells; a thread owns 8
-// consecutive cells (one segment lookup, vector loads and stores), the
-// block stages the tile's segment records in LDS.  Tiles are mapped to
+// The update kernel cuts the cells into tiles of kTileCells; a thread owns
+// 32 bytes of consecutive cells per array (8 floats or 4 doubles: two vector
+// groups, each one segment lookup, vector loads and stores), so a block has
+// 256 threads for float and 512 for double; the block stages the tile's
+// segment records in LDS.  Tiles are mapped to
 // blocks in XCD chunks (xcd_chunk_order) so the rows above and below a tile
 // are mostly read through the same L2.
 //
@@ -42,6 +44,14 @@ constexpr int kThreads = 256;
 constexpr int kCellsPerThread = 8;
 constexpr int kTileCells = kThreads * kCellsPerThread;
 constexpr int kSegCap = 512;
+// Threads of an update block: a tile of kTileCells at 32 bytes of cells per
+// thread (measured on the composite step: FP64 with 8 cells per thread
+// 201 us, with 4 cells 193 us; FP32 with 4 cells 124 us, with 8 cells 118 us,
+// with 16 cells 121 us).
+template <typename T>
+constexpr int seg_threads() {
+    return kTileCells * (int)sizeof(T) / 32;
+}
 constexpr int kXcds = 8;
 // Block -> tile order: the XCDs take chunks of kOrderChunk consecutive tiles
 // in turn (block b runs on XCD b % 8), so the rows above and below a tile are
@@ -321,16 +331,17 @@ __device__ __forceinline__ void upwind_thread(const SG& sg, int32_t s0, int32_t 
                                               const T* __restrict__ u, T* __restrict__ out, T a, T b, T dt, T dx,
                                               T dy, const T* __restrict__ inst_coef) {
     constexpr int G = Vec<T>::n;
+    constexpr int TH = seg_threads<T>();
     const __amdgpu_buffer_rsrc_t rs = field_rsrc(u, (int64_t)C * (int64_t)sizeof(T));
 #pragma unroll
-    for (int q = 0; q < kCellsPerThread / G; ++q) {
-        const int32_t c = ct + (q * kThreads + (int32_t)threadIdx.x) * G;
+    for (int q = 0; q < kTileCells / TH / G; ++q) {
+        const int32_t c = ct + (q * TH + (int32_t)threadIdx.x) * G;
         if (c < C) upwind_group<T, BATCH>(sg, s0, s1, c, C, rs, u, out, a, b, dt, dx, dy, inst_coef);
     }
 }
 
 template <typename T, bool BATCH>
-__global__ __launch_bounds__(kThreads) void upwind_seg_kernel(const int32_t* __restrict__ plan, int64_t ntiles,
+__global__ __launch_bounds__(seg_threads<T>()) void upwind_seg_kernel(const int32_t* __restrict__ plan, int64_t ntiles,
                                                               int32_t C, const int32_t* __restrict__ segs,
                                                               const T* __restrict__ u, T* __restrict__ out, T a,
                                                               T b, T dt, T dx, T dy,
@@ -343,7 +354,7 @@ __global__ __launch_bounds__(kThreads) void upwind_seg_kernel(const int32_t* __r
     const int32_t ct = (int32_t)(t * kTileCells);
     const int4* gs = (const int4*)segs;
     if (M <= kSegCap) {
-        for (int q = threadIdx.x; q < 2 * M; q += kThreads) s_seg[q] = gs[2 * (int64_t)s0 + q];
+        for (int q = threadIdx.x; q < 2 * M; q += seg_threads<T>()) s_seg[q] = gs[2 * (int64_t)s0 + q];
         __syncthreads();
         upwind_thread<T, BATCH>(LdsSegs{s_seg, s0}, s0, s1, ct, C, u, out, a, b, dt, dx, dy, inst_coef);
     } else {
@@ -576,7 +587,7 @@ int32_t launch_upwind_seg(const int32_t* plan, int64_t ntiles, int64_t C, const 
     if (grid > 0x7fffffffLL) return stx_fail(STX_EINVAL, "upwind_seg: too many tiles");
     if ((uintptr_t)u % 16 != 0 || (uintptr_t)out % 16 != 0)
         return stx_fail(STX_EINVAL, "upwind_seg: values and output must be 16-byte aligned");
-    hipLaunchKernelGGL((upwind_seg_kernel<T, BATCH>), dim3((unsigned)grid), dim3(kThreads), 0, stream, plan, ntiles,
+    hipLaunchKernelGGL((upwind_seg_kernel<T, BATCH>), dim3((unsigned)grid), dim3(seg_threads<T>()), 0, stream, plan, ntiles,
                        (int32_t)C, segs, (const T*)u, (T*)out, (T)a, (T)b, (T)dt, (T)dx, (T)dy, (const T*)inst_coef);
     return stx_launch_status("upwind_seg");
 }
