@@ -257,15 +257,16 @@ def _rows_to_oset(rows):
 
 
 def _flat_edge_pair(seed):
-    """Identity-row pair (2400 rows) whose tiles reach every branch of the
+    """Identity-row pair (4000 rows) whose tiles reach every branch of the
     flat tile kernel: flat tiles with negative and with large coordinates,
     tiles whose row bands overflow 31 bits (coordinates near +-1e9 and near
     the int32 limits: per-row fallback), a row longer than the LDS capacity
-    (per-row fallback), runs of empty rows, rows present in one operand only,
+    of every tile shape (per-row fallback), runs of empty rows (900 of them:
+    whole tiles without an interval), rows present in one operand only,
     touching runs."""
     rng = np.random.default_rng(seed)
     A, B = [], []
-    for r in range(2400):
+    for r in range(4000):
         base = 0
         if 200 <= r < 260:
             base = -5000
@@ -277,14 +278,14 @@ def _flat_edge_pair(seed):
             base = 3_000_000
         la = _row_runs(rng, base, int(rng.poisson(12)), r % 7 == 0)
         lb = _row_runs(rng, base + int(rng.integers(-20, 20)), int(rng.poisson(12)), r % 5 == 0)
-        if 300 <= r < 360:
+        if 300 <= r < 360 or 2500 <= r < 3400:
             la, lb = [], []
         elif 400 <= r < 420:
             lb = []
         elif 420 <= r < 440:
             la = []
         elif r == 1000:
-            la = _row_runs(rng, 0, 5000, False)
+            la = _row_runs(rng, 0, 9000, False)
         elif r == 2000:
             la = [(-2**31 + 1, -2**31 + 9), (2**31 - 20, 2**31 - 2)]
         elif r == 2001:
