@@ -627,7 +627,8 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(tile_waves_p
 // Flat tile kernel (identity row maps): the tile's rows merged as ONE sorted
 // sequence.
 //
-// Same tiles as setop_tile_kernel.  On staging, every value becomes
+// Tiles of whole rows as in setop_tile_kernel (up to one row per thread).
+// On staging, every value becomes
 //     key = (tile-local row << s) + (x - xmin)
 // where xmin and s (2^s > xmax - xmin) come from a block min / max over the
 // tile's begins and ends.  Each row then owns a disjoint band of keys, and
@@ -645,6 +646,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(tile_waves_p
 // the row of each output and of its predecessor.
 // A tile that does not fit (intervals beyond the LDS capacity, or
 // nr << s >= 2^31) is processed one wavefront per row instead.
+
 // Rows per flat tile: at most one per thread (the per-row fallback scans one
 // row count per thread).
 template <int TB>
