@@ -329,3 +329,32 @@ def test_flat_tile_edge_cases(sx, schedule, seed):
         assert_set_equal(to_host(sx.clip_interval_set(ds, width=60, height=R)), ox.clip(small, 60, R))
     finally:
         _core.set_group_override(None)
+
+
+@pytest.mark.parametrize("seed", [3, 4, 5])
+def test_default_schedule_heavy_tailed_rows(sx, seed):
+    """The default schedule on 3000 identity rows whose lengths are heavy
+    tailed (most rows short, a few thousand intervals long, some empty), with
+    negative coordinates: tiles of every size, rows that cross the tile
+    capacity, bit-exact against the oracle for the three operations."""
+    from subsetix_amd import _core
+
+    rng = np.random.default_rng(seed)
+    A, B = [], []
+    for r in range(3000):
+        na = int(min(rng.pareto(1.2) * 4, 12000))
+        nb = int(min(rng.pareto(1.2) * 4, 12000))
+        base = int(rng.integers(-3000, 3000))
+        A.append(_row_runs(rng, base, na, r % 11 == 0))
+        B.append(_row_runs(rng, base + int(rng.integers(-50, 50)), nb, r % 13 == 0))
+    a, b = _rows_to_oset(A), _rows_to_oset(B)
+    da, db = to_device(a), to_device(b)
+    for op, name in ((0, "union"), (1, "intersection"), (2, "difference")):
+        print("heavy-tailed seed", seed, name, a.n, b.n, flush=True)
+        exp = ox.setop(name, a, b)
+        out_off, ob, oe, ctx = _core.setop_raw(op, (da.begin, da.end, da.row_offsets),
+                                               (db.begin, db.end, db.row_offsets), nrows=da.row_count, group=0)
+        (total,) = _core.read_counts(ctx, ctx.words[1])
+        np.testing.assert_array_equal(out_off.cpu().numpy(), exp.off)
+        np.testing.assert_array_equal(ob[:total].cpu().numpy(), exp.beg)
+        np.testing.assert_array_equal(oe[:total].cpu().numpy(), exp.end)
