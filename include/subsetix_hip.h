@@ -256,8 +256,9 @@ int32_t stx_transfer(int32_t mode, int32_t dtype, int32_t acc64, int32_t group, 
                      const int32_t* d_co, int64_t d_R, const int32_t* s_rows, const int32_t* s_off, const int32_t* s_beg,
                      const int32_t* s_end, const int32_t* s_co, int64_t s_R, const void* src, void* dst,
                      hipStream_t stream);
-/* Segment schedule of the ratio-2 stx_transfer (same chains,
- * fields.py:570-612): stx_transfer_seg_count (per target interval counts,
+/* Segment schedule of stx_transfer (same chains, fields.py:570-612): mode 0
+ * ratio-2 prolong, 1 ratio-2 restrict, 2 ratio-1 copy (the ratio-1 prolong of
+ * stx_transfer).  stx_transfer_seg_count (per target interval counts,
  * their scan into seg_off[tN+1], total in *d_total), stx_transfer_seg_build
  * (records, plan = 2 int32 per tile of stx_transfer_tile_cells() target
  * cells), then stx_transfer_seg per step.  t_ord: row index of every target

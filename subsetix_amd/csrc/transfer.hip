@@ -126,19 +126,22 @@ void launch_restrict(const Geom& t, const Geom& d, const Geom& s, int32_t R, Acc
 }
 
 // ---------------------------------------------------------------------------
-// Segment form (ratio 2).  A per-(target, destination, source) table cuts the
-// target's intervals into segments over which the destination interval and
-// the source interval(s) do not change, so inside a segment, with c the
-// target's own cell index:
+// Segment form (ratio 2, and the ratio-1 copy).  A per-(target, destination,
+// source) table cuts the target's intervals into segments over which the
+// destination interval and the source interval(s) do not change, so inside a
+// segment, with c the target's own cell index:
 //   restrict: dst cell = dD + c, child (d, e) = sD_d + 2 c + e (d, e < 2),
 //             mask bit 2 d + e: the child exists (missing children read 0,
 //             which leaves an FP sum unchanged, exactly as skipping them);
-//   prolong:  dst cell = dD + c, src cell = sB + floor((xD + c) / 2).
+//   prolong:  dst cell = dD + c, src cell = sB + floor((xD + c) / 2);
+//   copy:     dst cell = dD + c, src cell = sD + c.
 // Touching intervals of a row have contiguous cells, so one base per source
 // row covers a segment even where it spans them.  Record (8 int32):
 //   restrict: c0, c1, dD, mask, sD0, sD1, 0, 0
 //   prolong:  c0, c1, dD, xD, sB, 0, 0, 0
-// (dD = kNone: destination cell absent; prolong sB = kNone: source absent).
+//   copy:     c0, c1, dD, 0, sD, 0, 0, 0
+// (dD = kNone: destination cell absent; prolong sB / copy sD = kNone: source
+// absent, the destination cell keeps its value).
 constexpr int32_t kNone = INT32_MIN;
 constexpr int kRec = 8;
 constexpr int kThreads = 256;
@@ -173,11 +176,15 @@ __device__ __forceinline__ int32_t ceil2(int32_t v) { return floordiv(v + 1, 2);
 
 // Smallest projected boundary of interval [b, e) above x.  Restrict (fine
 // interval seen in coarse columns): floor(b/2), ceil(b/2), floor(e/2),
-// ceil(e/2).  Prolong (coarse interval in fine columns): 2b, 2e.
+// ceil(e/2).  Prolong (coarse interval in fine columns): 2b, 2e.  Copy (same
+// columns): b, e.
 template <int MODE>
 __device__ __forceinline__ int32_t next_bound(int32_t b, int32_t e, int32_t x) {
     int32_t best = INT32_MAX;
-    if (MODE == 1) {
+    if (MODE == 2) {
+        if (b > x) best = b;
+        else if (e > x) best = e;
+    } else if (MODE == 1) {
         const int32_t c[4] = {floordiv(b, 2), ceil2(b), floordiv(e, 2), ceil2(e)};
 #pragma unroll
         for (int k = 0; k < 4; ++k)
@@ -206,18 +213,18 @@ __global__ void transfer_segments_kernel(Geom tg, const int32_t* __restrict__ t_
             jd = first_ending_after(dg.end, dg.off[k], jde, bT);
         }
     }
-    // source rows: restrict 2y, 2y + 1; prolong floor(y / 2)
+    // source rows: restrict 2y, 2y + 1; prolong floor(y / 2); copy y
     constexpr int NS = MODE == 1 ? 2 : 1;
     int32_t js[NS], jse[NS];
 #pragma unroll
     for (int d = 0; d < NS; ++d) {
         js[d] = jse[d] = 0;
-        const int32_t sy = MODE == 1 ? 2 * y + d : floordiv(y, 2);
+        const int32_t sy = MODE == 1 ? 2 * y + d : (MODE == 2 ? y : floordiv(y, 2));
         const int64_t k = find_row(sg.rows, sg.R, sy);
         if (k >= 0) {
             jse[d] = sg.off[k + 1];
             // first source interval whose projected end is above bT
-            const int32_t px = MODE == 1 ? 2 * bT : floordiv(bT, 2);
+            const int32_t px = MODE == 1 ? 2 * bT : (MODE == 2 ? bT : floordiv(bT, 2));
             js[d] = first_ending_after(sg.end, sg.off[k], jse[d], px);
         }
     }
@@ -233,7 +240,9 @@ __global__ void transfer_segments_kernel(Geom tg, const int32_t* __restrict__ t_
 #pragma unroll
         for (int d = 0; d < NS; ++d) {
             // skip source intervals whose projection ends at or before x
-            while (js[d] < jse[d] && (MODE == 1 ? ceil2(sg.end[js[d]]) <= x : 2 * sg.end[js[d]] <= x)) ++js[d];
+            while (js[d] < jse[d] &&
+                   (MODE == 1 ? ceil2(sg.end[js[d]]) <= x : (MODE == 2 ? sg.end[js[d]] : 2 * sg.end[js[d]]) <= x))
+                ++js[d];
             for (int q = 0; q < 2 && js[d] + q < jse[d]; ++q) {  // the interval and the next (touching) one
                 const int32_t nb = next_bound<MODE>(sg.beg[js[d] + q], sg.end[js[d] + q], x);
                 nx = nb < nx ? nb : nx;
@@ -264,6 +273,21 @@ __global__ void transfer_segments_kernel(Geom tg, const int32_t* __restrict__ t_
                     rec[4 + d] = sbase == kNone ? 0 : (int32_t)(sbase + 2 * (int64_t)x - 2 * (int64_t)c0);
                 }
                 rec[3] = mask;
+                rec[6] = 0;
+                rec[7] = 0;
+            } else if (MODE == 2) {
+                // source cell of target cell c: rec[4] + c (kNone: column absent)
+                int64_t sb = kNone;
+                for (int q = 0; q < 2 && js[0] + q < jse[0]; ++q) {
+                    const int32_t j = js[0] + q;
+                    if (sg.beg[j] <= x && x < sg.end[j]) {
+                        sb = (int64_t)sg.co[j] - sg.beg[j];
+                        break;
+                    }
+                }
+                rec[3] = 0;
+                rec[4] = sb == kNone ? kNone : (int32_t)(sb + x - c0);
+                rec[5] = 0;
                 rec[6] = 0;
                 rec[7] = 0;
             } else {
@@ -382,6 +406,10 @@ __device__ __forceinline__ void transfer_group(const SG& sg, int32_t s0, int32_t
         store_dw(dst + (g.dD + c), ov);
         return;
     }
+    if (MODE == 2 && c + G <= CT && c + G <= g.c1 && g.dD != kNone && g.a4 != kNone) {
+        store_dw(dst + (g.dD + c), load_dw<typename VecT<T>::type>(src + (g.a4 + c)));
+        return;
+    }
 #pragma unroll
     for (int v = 0; v < G; ++v) {
         const int32_t cc = c + v;
@@ -403,6 +431,9 @@ __device__ __forceinline__ void transfer_group(const SG& sg, int32_t s0, int32_t
                 }
             }
             dst[g.dD + cc] = (T)(acc * norm);
+        } else if (MODE == 2) {
+            if (g.a4 == kNone) continue;
+            dst[g.dD + cc] = src[g.a4 + cc];
         } else {
             if (g.a4 == kNone) continue;
             dst[g.dD + cc] = src[g.a4 + floordiv(g.a3 + cc, 2)];
@@ -542,8 +573,9 @@ extern "C" {
 
 int64_t stx_transfer_tile_cells(void) { return kTileCells; }
 
-// Segment table of a ratio-2 transfer (mode 0 prolong, 1 restrict) onto
-// target T (t_ord: row index of every target interval, tN intervals).
+// Segment table of a transfer onto target T (t_ord: row index of every target
+// interval, tN intervals): mode 0 ratio-2 prolong, 1 ratio-2 restrict, 2 copy
+// (ratio 1: dst(y, x) = src(y, x)).
 int32_t stx_transfer_seg_count(int32_t mode, const int32_t* t_rows, const int32_t* t_off, const int32_t* t_beg,
                                const int32_t* t_end, const int32_t* t_co, int64_t t_R, const int32_t* t_ord,
                                int64_t tN, const int32_t* d_rows, const int32_t* d_off, const int32_t* d_beg,
@@ -551,13 +583,16 @@ int32_t stx_transfer_seg_count(int32_t mode, const int32_t* t_rows, const int32_
                                const int32_t* s_off, const int32_t* s_beg, const int32_t* s_end, const int32_t* s_co,
                                int64_t s_R, int32_t* counts, int32_t* seg_off, int32_t* d_total, uint32_t* status,
                                void* scratch, int64_t scratch_bytes, uint32_t epoch, hipStream_t stream) {
-    if (mode != 0 && mode != 1) return stx_fail(STX_EINVAL, "transfer_seg: mode must be 0 or 1");
+    if (mode < 0 || mode > 2) return stx_fail(STX_EINVAL, "transfer_seg: mode must be 0, 1 or 2");
     if (tN < 0) return stx_fail(STX_EINVAL, "transfer_seg: bad sizes");
     Geom t{t_rows, t_off, t_beg, t_end, t_co, t_R};
     Geom d{d_rows, d_off, d_beg, d_end, d_co, d_R};
     Geom s{s_rows, s_off, s_beg, s_end, s_co, s_R};
     if (tN > 0) {
-        if (mode == 1)
+        if (mode == 2)
+            hipLaunchKernelGGL((transfer_segments_kernel<2, false>), dim3(nblocks256(tN)), dim3(256), 0, stream, t,
+                               t_ord, tN, d, s, counts, nullptr, nullptr);
+        else if (mode == 1)
             hipLaunchKernelGGL((transfer_segments_kernel<1, false>), dim3(nblocks256(tN)), dim3(256), 0, stream, t,
                                t_ord, tN, d, s, counts, nullptr, nullptr);
         else
@@ -576,14 +611,17 @@ int32_t stx_transfer_seg_build(int32_t mode, const int32_t* t_rows, const int32_
                                const int32_t* s_off, const int32_t* s_beg, const int32_t* s_end, const int32_t* s_co,
                                int64_t s_R, const int32_t* seg_off, int64_t nseg, int32_t* segs, int64_t CT,
                                int32_t* plan, int64_t ntiles, hipStream_t stream) {
-    if (mode != 0 && mode != 1) return stx_fail(STX_EINVAL, "transfer_seg: mode must be 0 or 1");
+    if (mode < 0 || mode > 2) return stx_fail(STX_EINVAL, "transfer_seg: mode must be 0, 1 or 2");
     if (ntiles != (CT + kTileCells - 1) / kTileCells) return stx_fail(STX_EINVAL, "transfer_seg: tile count");
     if (2 * CT + 16 > 0x7fffffffLL) return stx_fail(STX_EINVAL, "transfer_seg: too many target cells");
     Geom t{t_rows, t_off, t_beg, t_end, t_co, t_R};
     Geom d{d_rows, d_off, d_beg, d_end, d_co, d_R};
     Geom s{s_rows, s_off, s_beg, s_end, s_co, s_R};
     if (tN > 0) {
-        if (mode == 1)
+        if (mode == 2)
+            hipLaunchKernelGGL((transfer_segments_kernel<2, true>), dim3(nblocks256(tN)), dim3(256), 0, stream, t,
+                               t_ord, tN, d, s, nullptr, seg_off, segs);
+        else if (mode == 1)
             hipLaunchKernelGGL((transfer_segments_kernel<1, true>), dim3(nblocks256(tN)), dim3(256), 0, stream, t,
                                t_ord, tN, d, s, nullptr, seg_off, segs);
         else
@@ -598,8 +636,9 @@ int32_t stx_transfer_seg_build(int32_t mode, const int32_t* t_rows, const int32_
     return stx_launch_status("transfer_seg_build");
 }
 
-// Ratio-2 transfer with a segment table: dtype 4 / 8, acc64 as stx_transfer.
-// src_cells: number of cells of the source field (reads beyond are zero).
+// Transfer with a segment table (mode as the table's): dtype 4 / 8, acc64 as
+// stx_transfer (restrict only).  src_cells: number of cells of the source
+// field (reads beyond are zero).
 int32_t stx_transfer_seg(int32_t mode, int32_t dtype, int32_t acc64, const int32_t* plan, int64_t ntiles, int64_t CT,
                          const int32_t* segs, const void* src, int64_t src_cells, void* dst, hipStream_t stream) {
     if (ntiles <= 0 || CT <= 0) return STX_OK;
@@ -622,8 +661,15 @@ int32_t stx_transfer_seg(int32_t mode, int32_t dtype, int32_t acc64, const int32
             launch_transfer_seg<0, double, double>(plan, ntiles, CT, segs, src, src_cells, dst, 1.0, stream);
         else
             return stx_fail(STX_EINVAL, "transfer_seg: dtype must be 4 or 8");
+    } else if (mode == 2) {
+        if (dtype == 4)
+            launch_transfer_seg<2, float, float>(plan, ntiles, CT, segs, src, src_cells, dst, 1.0f, stream);
+        else if (dtype == 8)
+            launch_transfer_seg<2, double, double>(plan, ntiles, CT, segs, src, src_cells, dst, 1.0, stream);
+        else
+            return stx_fail(STX_EINVAL, "transfer_seg: dtype must be 4 or 8");
     } else {
-        return stx_fail(STX_EINVAL, "transfer_seg: mode must be 0 or 1");
+        return stx_fail(STX_EINVAL, "transfer_seg: mode must be 0, 1 or 2");
     }
     return stx_launch_status("transfer_seg");
 }

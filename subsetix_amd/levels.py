@@ -51,8 +51,9 @@ def _dtype_code(t: torch.Tensor) -> int:
 
 
 def transfer_plan(mode: int, target: IntervalSet, dst: IntervalField, src: IntervalField):
-    """Segment table of a ratio-2 transfer onto ``target`` (cached on the
-    target set, keyed by the destination and source geometries)."""
+    """Segment table of a transfer onto ``target`` (mode 0: ratio-2 prolong,
+    1: ratio-2 restrict, 2: ratio-1 copy), cached on the target set, keyed by
+    the destination and source geometries."""
     cache = target._meta.setdefault("transfer_plans", {})
     key = (mode, id(dst.interval_set), id(src.interval_set), id(dst.interval_cell_offsets),
            id(src.interval_cell_offsets))
@@ -95,16 +96,28 @@ _TRANSFER_SCHEDULE = "auto"
 
 def set_transfer_schedule(name: str) -> None:
     """'auto' (default: the autotuner's choice when it has one, else segment
-    tables at ratio 2), 'segments' (segment tables, ratio 2) or 'rows' (lane
-    group per target row, any ratio)."""
+    tables where they apply), 'segments' (segment tables: ratio 2, and the
+    ratio-1 prolong) or 'rows' (lane group per target row, any ratio)."""
     global _TRANSFER_SCHEDULE
     if name not in ("auto", "segments", "rows"):
         raise ValueError("schedule must be 'auto', 'segments' or 'rows'")
     _TRANSFER_SCHEDULE = name
 
 
-def _seg_possible(src: IntervalField, ratio: int) -> bool:
-    return ratio == 2 and src.values.numel() * src.values.element_size() < (1 << 32) - 256
+def _seg_mode(mode: int, ratio: int):
+    """Segment-table mode of a transfer, None where tables do not apply: the
+    ratio-2 prolong (0) and restrict (1), and the ratio-1 prolong as a copy
+    (2).  A ratio-1 restrict reads absent sources as 0 and stays on rows."""
+    if ratio == 2:
+        return mode
+    if ratio == 1 and mode == 0:
+        return 2
+    return None
+
+
+def _seg_possible(src: IntervalField, mode: int, ratio: int) -> bool:
+    return (_seg_mode(mode, ratio) is not None
+            and src.values.numel() * src.values.element_size() < (1 << 32) - 256)
 
 
 def _transfer(mode: int, target: IntervalSet, dst: IntervalField, src: IntervalField, ratio: int, acc64: int,
@@ -116,13 +129,14 @@ def _transfer(mode: int, target: IntervalSet, dst: IntervalField, src: IntervalF
     segment tables when they apply."""
     ctx = context(dst.values.device)
     dcode = _dtype_code(dst.values)
-    seg_ok = _seg_possible(src, ratio)
+    seg_ok = _seg_possible(src, mode, ratio)
+    smode = _seg_mode(mode, ratio)
     group_cells = int(dst.values.numel()) if mode == 0 else int(src.values.numel())
 
     def launch(choice, d: IntervalField) -> None:
         if choice == "seg":
-            plan, ntiles, cells, segs = transfer_plan(mode, target, d, src)
-            check(_lib.lib().stx_transfer_seg(mode, dcode, acc64, ptr(plan), ntiles, cells, ptr(segs),
+            plan, ntiles, cells, segs = transfer_plan(smode, target, d, src)
+            check(_lib.lib().stx_transfer_seg(smode, dcode, acc64, ptr(plan), ntiles, cells, ptr(segs),
                                               ptr(src.values), int(src.values.numel()), ptr(d.values), ctx.stream),
                   "prolong_into" if mode == 0 else "restrict_into")
             return

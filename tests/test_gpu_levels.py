@@ -86,6 +86,20 @@ def _prolong_restrict_case(sx, dt, touching):
             if (y, x) in tcells and (y // 2, x // 2) in cc:
                 exp[co[i] + k] = cc[(y // 2, x // 2)]
     np.testing.assert_array_equal(ff.values.cpu().numpy(), exp)
+    # ratio-1 prolong (a copy): target cells outside the source keep their value
+    other = random_canonical(rng, 60, 120, mean_n=5, rows=np.arange(60), touching=not touching)
+    ov = rng.random(int(ox.cell_offsets(other)[-1])).astype(npdt)
+    of = _field(sx, other, ov, tdt)
+    target_1 = ox.intersection(random_canonical(rng, 60, 120, mean_n=6, rows=np.arange(60)), fine)
+    prolong_into(to_device(target_1), ff, of, 1)
+    oc = _cells(other, ov)
+    t1 = set(ox.cells(target_1))
+    for i in range(fine.n):
+        for k in range(int(fine.end[i] - fine.beg[i])):
+            y, x = int(ir[i]), int(fine.beg[i]) + k
+            if (y, x) in t1 and (y, x) in oc:
+                exp[co[i] + k] = oc[(y, x)]
+    np.testing.assert_array_equal(ff.values.cpu().numpy(), exp)
     # restrict onto a coarse target
     target_c = ox.intersection(random_canonical(rng, 30, 60, mean_n=3, rows=np.arange(30)), coarse)
     fv2 = rng.random(fv.size).astype(npdt)
