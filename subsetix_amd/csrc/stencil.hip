@@ -245,57 +245,38 @@ __device__ __forceinline__ typename Vec<T>::type load_dword_aligned(const T* p) 
     return v;
 }
 
-// One 16-byte group of G = 16 / sizeof(T) cells [c, c + G), c a multiple of
-// G.  Lanes of a wave own consecutive groups, so every vector load of the
-// wave covers one contiguous 1 KiB span.  Fast path (the group inside one
-// segment): one aligned load for the own values, one dword-aligned vector
-// load each for the cells below and above, two scalar loads for the cells
-// beside the group.  Otherwise a per-cell walk with scalar gathers.
+template <typename T>
+__device__ __forceinline__ typename Vec<T>::type gather_vec(__amdgpu_buffer_rsrc_t r, int32_t idx) {
+    return __builtin_bit_cast(typename Vec<T>::type,
+                              __builtin_amdgcn_raw_buffer_load_b128(r, idx * (int)sizeof(T), 0, 0));
+}
+
+// Coefficients a, b, dt, dx, dy of instance `inst` of a stack.  The first
+// active lane's instance is read through the scalar cache, off the vector
+// memory path that the field streams through (a wave rarely spans two
+// instances); lanes of another instance read theirs.
+template <typename T>
+__device__ __forceinline__ void instance_coef(const T* __restrict__ inst_coef, int32_t inst, T* k) {
+    typedef const __attribute__((address_space(4))) T ConstT;
+    const int32_t i0 = __builtin_amdgcn_readfirstlane(inst);
+    ConstT* q0 = (ConstT*)(inst_coef) + 5 * (int64_t)i0;
+#pragma unroll
+    for (int j = 0; j < 5; ++j) k[j] = q0[j];
+    if (inst != i0) {
+        const T* q = inst_coef + 5 * (int64_t)inst;
+#pragma unroll
+        for (int j = 0; j < 5; ++j) k[j] = q[j];
+    }
+}
+
+// Cells [c, c + G) crossing a segment end or the field end: a per-cell walk
+// with scalar gathers.
 template <typename T, bool BATCH, class SG>
-__device__ __forceinline__ void upwind_group(const SG& sg, int32_t s0, int32_t s1, int32_t c, int32_t C,
+__device__ __forceinline__ void upwind_cells(const SG& sg, int32_t s, int32_t c, int32_t C,
                                              __amdgpu_buffer_rsrc_t rs, const T* __restrict__ u, T* __restrict__ out,
                                              T a, T b, T dt, T dx, T dy, const T* __restrict__ inst_coef) {
-    using VT = typename Vec<T>::type;
     constexpr int G = Vec<T>::n;
-    int32_t lo = s0, hi = s1;
-    while (hi - lo > 1) {
-        const int32_t mid = (lo + hi) >> 1;
-        if (sg.c0(mid) <= c) lo = mid; else hi = mid;
-    }
-    int32_t s = lo;
     Seg g = sg.get(s);
-    if (c + G <= C && c + G <= g.cend) {
-        if (BATCH) {
-            const T* q = inst_coef + 5 * (int64_t)g.inst;
-            a = q[0];
-            b = q[1];
-            dt = q[2];
-            dx = q[3];
-            dy = q[4];
-            if (dx == (T)0) return;  // retired instance
-        }
-        const VT cv = *(const VT*)(u + c);
-        VT dv = load_dword_aligned(u + (g.dd != kNone ? c + g.dd : c));
-        VT uv = load_dword_aligned(u + (g.ud != kNone ? c + g.ud : c));
-        const T lv = gather<T>(rs, c - 1);
-        const T rv = gather<T>(rs, c + G);
-        if (g.dd == kNone) dv = (VT)(T)0;
-        if (g.ud == kNone) uv = (VT)(T)0;
-        const bool inner = g.flags & 2;
-        const bool lfirst = c == g.c0 ? (g.flags & 1) : inner;
-        const bool rlast = c + G == g.cend ? (g.flags & 4) : inner;
-        VT ov;
-#pragma unroll
-        for (int v = 0; v < G; ++v) {
-            const bool lok = v == 0 ? lfirst : inner;
-            const bool rok = v == G - 1 ? rlast : inner;
-            const T left = lok ? (v == 0 ? lv : cv[v - 1]) : (T)0;
-            const T right = rok ? (v == G - 1 ? rv : cv[v + 1]) : (T)0;
-            ov[v] = upwind_cell<T>(cv[v], left, right, dv[v], uv[v], a, b, dt, dx, dy);
-        }
-        *(VT*)(out + c) = ov;
-        return;
-    }
 #pragma unroll
     for (int v = 0; v < G; ++v) {
         const int32_t cc = c + v;
@@ -326,20 +307,119 @@ __device__ __forceinline__ void upwind_group(const SG& sg, int32_t s0, int32_t s
     }
 }
 
+// One group of G = 16 / sizeof(T) cells [c, c + G), c a multiple of G, in
+// two steps.  issue(): the group's segment (binary search over the tile's
+// segments), its rows below and above (one dword-aligned 16-byte load each;
+// reads past the field return 0) and its instance's coefficients.
+// finish(): a group inside one segment (the fast path) computes from its own
+// values cv (with lv / rv, the cells beside it, loaded at the kernel's
+// start) and stores one vector; other groups walk their cells.
+template <typename T>
+struct UpwindGroup {
+    using VT = typename Vec<T>::type;
+    static constexpr int G = Vec<T>::n;
+    int32_t s;
+    uint32_t bits;  // 1: fast path, 2: left neighbour of the first cell, 4: right of the last, 8: inner
+    VT dv, uv;
+    T k[5];
+
+    template <bool BATCH, class SG>
+    __device__ __forceinline__ void issue(const SG& sg, int32_t s0, int32_t s1, int32_t c, int32_t C,
+                                          __amdgpu_buffer_rsrc_t rs, T a, T b, T dt, T dx, T dy,
+                                          const T* __restrict__ inst_coef) {
+        int32_t lo = s0, hi = s1;
+        while (hi - lo > 1) {
+            const int32_t mid = (lo + hi) >> 1;
+            if (sg.c0(mid) <= c) lo = mid; else hi = mid;
+        }
+        s = lo;
+        const Seg g = sg.get(lo);
+        const bool inner = g.flags & 2;
+        const bool lfirst = c == g.c0 ? (g.flags & 1) : inner;
+        const bool rlast = c + G == g.cend ? (g.flags & 4) : inner;
+        const bool fast = c + G <= C && c + G <= g.cend;
+        bits = (fast ? 1u : 0u) | (lfirst ? 2u : 0u) | (rlast ? 4u : 0u) | (inner ? 8u : 0u);
+        dv = gather_vec<T>(rs, g.dd != kNone ? c + g.dd : -G);  // -G: no row below, reads 0
+        uv = gather_vec<T>(rs, g.ud != kNone ? c + g.ud : -G);
+        if (BATCH) {
+            instance_coef<T>(inst_coef, g.inst, k);
+        } else {
+            k[0] = a;
+            k[1] = b;
+            k[2] = dt;
+            k[3] = dx;
+            k[4] = dy;
+        }
+    }
+
+    template <bool BATCH, class SG>
+    __device__ __forceinline__ void finish(const SG& sg, int32_t c, int32_t C, __amdgpu_buffer_rsrc_t rs,
+                                           const T* __restrict__ u, T* __restrict__ out, const VT& cv, T lv, T rv,
+                                           T a, T b, T dt, T dx, T dy, const T* __restrict__ inst_coef) const {
+        if (!(bits & 1u)) {
+            upwind_cells<T, BATCH>(sg, s, c, C, rs, u, out, a, b, dt, dx, dy, inst_coef);
+            return;
+        }
+        const bool inner = bits & 8u;
+        VT ov;
+#pragma unroll
+        for (int v = 0; v < G; ++v) {
+            const bool lok = v == 0 ? (bool)(bits & 2u) : inner;
+            const bool rok = v == G - 1 ? (bool)(bits & 4u) : inner;
+            const T left = lok ? (v == 0 ? lv : cv[v - 1]) : (T)0;
+            const T right = rok ? (v == G - 1 ? rv : cv[v + 1]) : (T)0;
+            ov[v] = upwind_cell<T>(cv[v], left, right, dv[v], uv[v], k[0], k[1], k[2], k[3], k[4]);
+        }
+        if (!BATCH || k[3] != (T)0) *(VT*)(out + c) = ov;  // dx = 0: retired instance
+    }
+};
+
+// A thread's NG groups, c = ct + (q TH + tid) G: lanes of a wave own
+// consecutive groups, so every vector load of the wave covers one contiguous
+// 1 KiB span.  FP32 issues every group's loads before finishing any (their
+// latencies overlap); FP64 finishes each group in turn, since holding both
+// groups' state needs more than 64 VGPRs and halves the resident waves
+// (measured on 512 stacked instances, 327M cells: FP32 level step 751 ->
+// 707 us with the overlap, FP64 1279 -> 1402 us).
 template <typename T, bool BATCH, class SG>
 __device__ __forceinline__ void upwind_thread(const SG& sg, int32_t s0, int32_t s1, int32_t ct, int32_t C,
-                                              const T* __restrict__ u, T* __restrict__ out, T a, T b, T dt, T dx,
-                                              T dy, const T* __restrict__ inst_coef) {
+                                              __amdgpu_buffer_rsrc_t rs, const T* __restrict__ u,
+                                              T* __restrict__ out, T a, T b, T dt, T dx, T dy,
+                                              const T* __restrict__ inst_coef, const typename Vec<T>::type* cv,
+                                              const T* lv, const T* rv) {
     constexpr int G = Vec<T>::n;
     constexpr int TH = seg_threads<T>();
-    const __amdgpu_buffer_rsrc_t rs = field_rsrc(u, (int64_t)C * (int64_t)sizeof(T));
+    constexpr int NG = kTileCells / TH / G;
+    if constexpr (sizeof(T) == 4) {
+        UpwindGroup<T> grp[NG];
 #pragma unroll
-    for (int q = 0; q < kTileCells / TH / G; ++q) {
-        const int32_t c = ct + (q * TH + (int32_t)threadIdx.x) * G;
-        if (c < C) upwind_group<T, BATCH>(sg, s0, s1, c, C, rs, u, out, a, b, dt, dx, dy, inst_coef);
+        for (int q = 0; q < NG; ++q)
+            grp[q].template issue<BATCH>(sg, s0, s1, ct + (q * TH + (int32_t)threadIdx.x) * G, C, rs, a, b, dt, dx,
+                                         dy, inst_coef);
+#pragma unroll
+        for (int q = 0; q < NG; ++q) {
+            const int32_t c = ct + (q * TH + (int32_t)threadIdx.x) * G;
+            if (c < C)
+                grp[q].template finish<BATCH>(sg, c, C, rs, u, out, cv[q], lv[q], rv[q], a, b, dt, dx, dy,
+                                              inst_coef);
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < NG; ++q) {
+            const int32_t c = ct + (q * TH + (int32_t)threadIdx.x) * G;
+            if (c < C) {
+                UpwindGroup<T> grp;
+                grp.template issue<BATCH>(sg, s0, s1, c, C, rs, a, b, dt, dx, dy, inst_coef);
+                grp.template finish<BATCH>(sg, c, C, rs, u, out, cv[q], lv[q], rv[q], a, b, dt, dx, dy, inst_coef);
+            }
+        }
     }
 }
 
+// One block per tile of kTileCells cells.  Every thread first loads its
+// groups' own values and the cells beside them (they do not depend on the
+// segments, so their HBM latency overlaps the segment staging); the tile's
+// segments then go to LDS (or, past kSegCap, are searched in global memory).
 template <typename T, bool BATCH>
 __global__ __launch_bounds__(seg_threads<T>()) void upwind_seg_kernel(const int32_t* __restrict__ plan, int64_t ntiles,
                                                               int32_t C, const int32_t* __restrict__ segs,
@@ -349,16 +429,30 @@ __global__ __launch_bounds__(seg_threads<T>()) void upwind_seg_kernel(const int3
     __shared__ int4 s_seg[2 * kSegCap];
     const int64_t t = xcd_chunk_order(blockIdx.x);
     if (t >= ntiles) return;
+    constexpr int G = Vec<T>::n;
+    constexpr int TH = seg_threads<T>();
+    constexpr int NG = kTileCells / TH / G;
+    const int32_t ct = (int32_t)(t * kTileCells);
+    const __amdgpu_buffer_rsrc_t rs = field_rsrc(u, (int64_t)C * (int64_t)sizeof(T));
+    typename Vec<T>::type cv[NG];
+    T lv[NG], rv[NG];
+#pragma unroll
+    for (int q = 0; q < NG; ++q) {  // reads past C return 0
+        const int32_t c = ct + (q * TH + (int32_t)threadIdx.x) * G;
+        cv[q] = gather_vec<T>(rs, c);
+        lv[q] = gather<T>(rs, c - 1);
+        rv[q] = gather<T>(rs, c + G);
+    }
     const int32_t s0 = plan[2 * t], s1 = plan[2 * t + 1];
     const int32_t M = s1 - s0;
-    const int32_t ct = (int32_t)(t * kTileCells);
     const int4* gs = (const int4*)segs;
     if (M <= kSegCap) {
-        for (int q = threadIdx.x; q < 2 * M; q += seg_threads<T>()) s_seg[q] = gs[2 * (int64_t)s0 + q];
+        for (int q = threadIdx.x; q < 2 * M; q += TH) s_seg[q] = gs[2 * (int64_t)s0 + q];
         __syncthreads();
-        upwind_thread<T, BATCH>(LdsSegs{s_seg, s0}, s0, s1, ct, C, u, out, a, b, dt, dx, dy, inst_coef);
+        upwind_thread<T, BATCH>(LdsSegs{s_seg, s0}, s0, s1, ct, C, rs, u, out, a, b, dt, dx, dy, inst_coef, cv, lv,
+                                rv);
     } else {
-        upwind_thread<T, BATCH>(GlobalSegs{gs}, s0, s1, ct, C, u, out, a, b, dt, dx, dy, inst_coef);
+        upwind_thread<T, BATCH>(GlobalSegs{gs}, s0, s1, ct, C, rs, u, out, a, b, dt, dx, dy, inst_coef, cv, lv, rv);
     }
 }
 
