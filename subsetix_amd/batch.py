@@ -194,7 +194,10 @@ class BatchedAMR:
             tmp1 = self._level_field(1, l1, cur0, old, old_sets, ghost=None)
             comp = binary_op(D[1], l1, "difference")
             eroded = binary_op(l1, dilate_interval_set(comp, 1, 1), "difference")
-            tag1 = self._tag(tmp1, 1)
+            # a segment table for the tag (every level-1 cell lies inside its
+            # instance's width): bit-mask passes instead of row walks
+            plan1 = build_stencil_plan(l1, tmp1.interval_cell_offsets, 0, self._row_inst(l1, 1), self.inst_w[1])
+            tag1 = self._tag(tmp1, 1, plan1)
             refine1 = binary_op(dilate_interval_set(tag1, p, p) if p > 0 else tag1, eroded, "intersection")
             l2 = prolong_set(refine1, R)
             sets.append(l2)

@@ -239,13 +239,6 @@ __device__ __forceinline__ double gather<double>(__amdgpu_buffer_rsrc_t r, int32
 }
 
 template <typename T>
-__device__ __forceinline__ typename Vec<T>::type load_dword_aligned(const T* p) {
-    typename Vec<T>::type v;
-    __builtin_memcpy(&v, __builtin_assume_aligned(p, 4), 16);
-    return v;
-}
-
-template <typename T>
 __device__ __forceinline__ typename Vec<T>::type gather_vec(__amdgpu_buffer_rsrc_t r, int32_t idx) {
     return __builtin_bit_cast(typename Vec<T>::type,
                               __builtin_amdgcn_raw_buffer_load_b128(r, idx * (int)sizeof(T), 0, 0));
@@ -486,104 +479,103 @@ __device__ __forceinline__ bool grad_tag_cell(T center, T left, T right, T down,
     return sqrt(gx2 + gy2) >= t;
 }
 
-// Tag and left-neighbour bits of the group [c, c + G) (bit v = cell c + v).
-template <typename T, class SG>
-__device__ __forceinline__ void tag_group(const SG& sg, int32_t s0, int32_t s1, int32_t c, int32_t C,
-                                          __amdgpu_buffer_rsrc_t rs, const T* __restrict__ u, T thr,
-                                          uint32_t& tags, uint32_t& lex, bool& outside) {
+// Tag and left-neighbour bits of one group [c, c + G) (bit v = cell c + v),
+// in two steps as UpwindGroup: issue() finds the segment and loads the rows
+// below and above; finish() computes from the own values cv and the cells
+// beside the group (lv, rv), or walks the cells of a group that crosses a
+// segment end or the field end.
+template <typename T>
+struct TagGroup {
     using VT = typename Vec<T>::type;
-    constexpr int G = Vec<T>::n;
-    int32_t lo = s0, hi = s1;
-    while (hi - lo > 1) {
-        const int32_t mid = (lo + hi) >> 1;
-        if (sg.c0(mid) <= c) lo = mid; else hi = mid;
-    }
-    int32_t s = lo;
-    Seg g = sg.get(s);
-    tags = 0u;
-    lex = 0u;
-    if (c + G <= C && c + G <= g.cend) {
-        const VT cv = *(const VT*)(u + c);
-        const VT dv = load_dword_aligned(u + (g.dd != kNone ? c + g.dd : c));
-        const VT uv = load_dword_aligned(u + (g.ud != kNone ? c + g.ud : c));
-        const T lv = gather<T>(rs, c - 1);
-        const T rv = gather<T>(rs, c + G);
+    static constexpr int G = Vec<T>::n;
+    int32_t s;
+    uint32_t bits;  // 1: fast path, 2: left neighbour of the first cell, 4: right of the last, 8: inner,
+                    // 16: row below, 32: row above
+    VT dv, uv;
+
+    template <class SG>
+    __device__ __forceinline__ void issue(const SG& sg, int32_t s0, int32_t s1, int32_t c, int32_t C,
+                                          __amdgpu_buffer_rsrc_t rs) {
+        int32_t lo = s0, hi = s1;
+        while (hi - lo > 1) {
+            const int32_t mid = (lo + hi) >> 1;
+            if (sg.c0(mid) <= c) lo = mid; else hi = mid;
+        }
+        s = lo;
+        const Seg g = sg.get(lo);
         const bool inner = g.flags & 2;
-        outside |= !inner;
         const bool lfirst = c == g.c0 ? (g.flags & 1) : inner;
         const bool rlast = c + G == g.cend ? (g.flags & 4) : inner;
-        const bool hd = g.dd != kNone, hu = g.ud != kNone;
+        const bool fast = c + G <= C && c + G <= g.cend;
+        bits = (fast ? 1u : 0u) | (lfirst ? 2u : 0u) | (rlast ? 4u : 0u) | (inner ? 8u : 0u) |
+               (g.dd != kNone ? 16u : 0u) | (g.ud != kNone ? 32u : 0u);
+        dv = gather_vec<T>(rs, g.dd != kNone ? c + g.dd : -G);  // -G: no row below, reads 0
+        uv = gather_vec<T>(rs, g.ud != kNone ? c + g.ud : -G);
+    }
+
+    template <class SG>
+    __device__ __forceinline__ void finish(const SG& sg, int32_t c, int32_t C, __amdgpu_buffer_rsrc_t rs,
+                                           const T* __restrict__ u, const VT& cv, T lv, T rv, T thr, uint32_t& tags,
+                                           uint32_t& lex, bool& outside) const {
+        tags = 0u;
+        lex = 0u;
+        if (bits & 1u) {
+            const bool inner = bits & 8u;
+            outside |= !inner;
+            const bool hd = bits & 16u, hu = bits & 32u;
+#pragma unroll
+            for (int v = 0; v < G; ++v) {
+                const bool lok = v == 0 ? (bool)(bits & 2u) : inner;
+                const bool rok = v == G - 1 ? (bool)(bits & 4u) : inner;
+                const T left = v == 0 ? lv : cv[v - 1];
+                const T right = v == G - 1 ? rv : cv[v + 1];
+                if (grad_tag_cell<T>(cv[v], left, right, dv[v], uv[v], lok, rok, hd, hu, thr)) tags |= 1u << v;
+                if (lok) lex |= 1u << v;
+            }
+            return;
+        }
+        int32_t si = s;
+        Seg g = sg.get(si);
 #pragma unroll
         for (int v = 0; v < G; ++v) {
-            const bool lok = v == 0 ? lfirst : inner;
-            const bool rok = v == G - 1 ? rlast : inner;
-            const T left = v == 0 ? lv : cv[v - 1];
-            const T right = v == G - 1 ? rv : cv[v + 1];
-            if (grad_tag_cell<T>(cv[v], left, right, dv[v], uv[v], lok, rok, hd, hu, thr)) tags |= 1u << v;
-            if (lok) lex |= 1u << v;
-        }
-        return;
-    }
-#pragma unroll
-    for (int v = 0; v < G; ++v) {
-        const int32_t cc = c + v;
-        if (cc < C) {
-            if (cc >= g.cend) {
-                ++s;
-                g = sg.get(s);
+            const int32_t cc = c + v;
+            if (cc < C) {
+                if (cc >= g.cend) {
+                    ++si;
+                    g = sg.get(si);
+                }
+                outside |= !(g.flags & 2);
+                const bool lok = (cc == g.c0) ? (g.flags & 1) : (g.flags & 2);
+                const bool rok = (cc + 1 == g.cend) ? (g.flags & 4) : (g.flags & 2);
+                const T center = u[cc];
+                const T left = gather<T>(rs, cc - 1);
+                const T right = gather<T>(rs, cc + 1);
+                const T down = gather<T>(rs, g.dd != kNone ? cc + g.dd : -1);
+                const T up = gather<T>(rs, g.ud != kNone ? cc + g.ud : -1);
+                if (grad_tag_cell<T>(center, left, right, down, up, lok, rok, g.dd != kNone, g.ud != kNone, thr))
+                    tags |= 1u << v;
+                if (lok) lex |= 1u << v;
             }
-            outside |= !(g.flags & 2);
-            const bool lok = (cc == g.c0) ? (g.flags & 1) : (g.flags & 2);
-            const bool rok = (cc + 1 == g.cend) ? (g.flags & 4) : (g.flags & 2);
-            const T center = u[cc];
-            const T left = gather<T>(rs, cc - 1);
-            const T right = gather<T>(rs, cc + 1);
-            const T down = gather<T>(rs, g.dd != kNone ? cc + g.dd : -1);
-            const T up = gather<T>(rs, g.ud != kNone ? cc + g.ud : -1);
-            if (grad_tag_cell<T>(center, left, right, down, up, lok, rok, g.dd != kNone, g.ud != kNone, thr))
-                tags |= 1u << v;
-            if (lok) lex |= 1u << v;
         }
     }
-}
+};
 
-// Pass 1.  Lanes own consecutive G-cell groups, so 32 / G consecutive lanes
-// hold one 32-cell word: their bits are OR-reduced across the lanes and the
-// first lane of each word writes it.  Every lane of a block takes part in the
-// reductions (lanes past the last cell contribute zero bits).
-template <typename T>
-__global__ __launch_bounds__(kThreads) void tag_bits_kernel(const int32_t* __restrict__ plan, int64_t ntiles,
-                                                            int32_t C, const int32_t* __restrict__ segs,
-                                                            const T* __restrict__ u, T thr,
-                                                            uint32_t* __restrict__ tag_w, uint32_t* __restrict__ lex_w,
-                                                            int64_t W, uint32_t* __restrict__ status) {
+// One thread's NG groups, then the 32-cell words: 32 / G consecutive lanes
+// hold one word, their bits are OR-reduced across the lanes and the first
+// lane of each word writes it.  Every lane takes part in the reductions
+// (lanes past the last cell contribute zero bits).  FP32 issues both groups'
+// loads before finishing either; FP64 (four groups) one group at a time.
+template <typename T, class SG>
+__device__ __forceinline__ void tag_thread(const SG& sg, int32_t s0, int32_t s1, int32_t ct, int32_t C,
+                                           __amdgpu_buffer_rsrc_t rs, const T* __restrict__ u, T thr,
+                                           const typename Vec<T>::type* cv, const T* lv, const T* rv,
+                                           uint32_t* __restrict__ tag_w, uint32_t* __restrict__ lex_w, int64_t W,
+                                           bool& outside) {
     constexpr int G = Vec<T>::n;
     constexpr int LPW = 32 / G;  // lanes per word
-    __shared__ int4 s_seg[2 * kSegCap];
-    const int64_t t = xcd_chunk_order(blockIdx.x);
-    if (t >= ntiles) return;
-    const int32_t s0 = plan[2 * t], s1 = plan[2 * t + 1];
-    const int32_t M = s1 - s0;
-    const int32_t ct = (int32_t)(t * kTileCells);
-    const int4* gs = (const int4*)segs;
-    const bool lds = M <= kSegCap;
-    if (lds) {
-        for (int q = threadIdx.x; q < 2 * M; q += kThreads) s_seg[q] = gs[2 * (int64_t)s0 + q];
-        __syncthreads();
-    }
-    const __amdgpu_buffer_rsrc_t rs = field_rsrc(u, (int64_t)C * (int64_t)sizeof(T));
+    constexpr int NG = kCellsPerThread / G;
     const int lane = lane_id();
-    bool outside = false;
-#pragma unroll
-    for (int q = 0; q < kCellsPerThread / G; ++q) {
-        const int32_t c = ct + (q * kThreads + (int32_t)threadIdx.x) * G;
-        uint32_t tg = 0u, lx = 0u;
-        if (c < C) {
-            if (lds)
-                tag_group<T>(LdsSegs{s_seg, s0}, s0, s1, c, C, rs, u, thr, tg, lx, outside);
-            else
-                tag_group<T>(GlobalSegs{gs}, s0, s1, c, C, rs, u, thr, tg, lx, outside);
-        }
+    auto word = [&](int32_t c, uint32_t tg, uint32_t lx) {
         const int sh = (lane % LPW) * G;
         uint32_t tw = tg << sh, lw = lx << sh;
 #pragma unroll
@@ -596,6 +588,67 @@ __global__ __launch_bounds__(kThreads) void tag_bits_kernel(const int32_t* __res
             tag_w[w] = tw;
             lex_w[w] = lw;
         }
+    };
+    if constexpr (sizeof(T) == 4) {
+        TagGroup<T> grp[NG];
+#pragma unroll
+        for (int q = 0; q < NG; ++q) grp[q].issue(sg, s0, s1, ct + (q * kThreads + (int32_t)threadIdx.x) * G, C, rs);
+#pragma unroll
+        for (int q = 0; q < NG; ++q) {
+            const int32_t c = ct + (q * kThreads + (int32_t)threadIdx.x) * G;
+            uint32_t tg = 0u, lx = 0u;
+            if (c < C) grp[q].finish(sg, c, C, rs, u, cv[q], lv[q], rv[q], thr, tg, lx, outside);
+            word(c, tg, lx);
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < NG; ++q) {
+            const int32_t c = ct + (q * kThreads + (int32_t)threadIdx.x) * G;
+            uint32_t tg = 0u, lx = 0u;
+            if (c < C) {
+                TagGroup<T> grp;
+                grp.issue(sg, s0, s1, c, C, rs);
+                grp.finish(sg, c, C, rs, u, cv[q], lv[q], rv[q], thr, tg, lx, outside);
+            }
+            word(c, tg, lx);
+        }
+    }
+}
+
+// Pass 1.  Own values and the cells beside each group load first (they do
+// not depend on the segments); the tile's segments then go to LDS.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void tag_bits_kernel(const int32_t* __restrict__ plan, int64_t ntiles,
+                                                            int32_t C, const int32_t* __restrict__ segs,
+                                                            const T* __restrict__ u, T thr,
+                                                            uint32_t* __restrict__ tag_w, uint32_t* __restrict__ lex_w,
+                                                            int64_t W, uint32_t* __restrict__ status) {
+    constexpr int G = Vec<T>::n;
+    constexpr int NG = kCellsPerThread / G;
+    __shared__ int4 s_seg[2 * kSegCap];
+    const int64_t t = xcd_chunk_order(blockIdx.x);
+    if (t >= ntiles) return;
+    const int32_t ct = (int32_t)(t * kTileCells);
+    const __amdgpu_buffer_rsrc_t rs = field_rsrc(u, (int64_t)C * (int64_t)sizeof(T));
+    typename Vec<T>::type cv[NG];
+    T lv[NG], rv[NG];
+#pragma unroll
+    for (int q = 0; q < NG; ++q) {  // reads past C return 0
+        const int32_t c = ct + (q * kThreads + (int32_t)threadIdx.x) * G;
+        cv[q] = gather_vec<T>(rs, c);
+        lv[q] = gather<T>(rs, c - 1);
+        rv[q] = gather<T>(rs, c + G);
+    }
+    const int32_t s0 = plan[2 * t], s1 = plan[2 * t + 1];
+    const int32_t M = s1 - s0;
+    const int4* gs = (const int4*)segs;
+    bool outside = false;
+    if (M <= kSegCap) {
+        for (int q = threadIdx.x; q < 2 * M; q += kThreads) s_seg[q] = gs[2 * (int64_t)s0 + q];
+        __syncthreads();
+        tag_thread<T>(LdsSegs{s_seg, s0}, s0, s1, ct, C, rs, u, thr, cv, lv, rv, tag_w, lex_w, W, outside);
+    } else {
+        tag_thread<T>(GlobalSegs{gs}, s0, s1, ct, C, rs, u, thr, cv, lv, rv, tag_w, lex_w, W, outside);
     }
     if (outside) atomicOr(status, STX_FLAG_DOMAIN);
 }
