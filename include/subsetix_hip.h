@@ -286,7 +286,8 @@ int32_t stx_transfer_seg(int32_t mode, int32_t dtype, int32_t acc64, const int32
 int32_t stx_row_instance(const int32_t* rows, int64_t R, const int32_t* bases, int32_t ninst, int32_t* out,
                          hipStream_t stream);
 /* FP64 mass per instance: out[i] = scale[i] * sum of u over cells
- * [cstart[i], cstart[i+1]) with a fixed reduction order. */
+ * [cstart[i], cstart[i+1]) with a fixed reduction order that depends only on
+ * the instance's cell count (the same mass in any stack). */
 int32_t stx_instance_mass(int32_t dtype, const void* u, const int64_t* cstart, int32_t ninst, const double* scale,
                           double* out, hipStream_t stream);
 

@@ -23,18 +23,44 @@ __global__ void row_instance_kernel(const int32_t* __restrict__ rows, int64_t R,
 }
 
 // Block per instance: cells [cstart[i], cstart[i+1]) summed with a fixed
-// order (strided per thread, then a fixed tree), times scale[i].
+// order that depends only on the instance's cell count n (so an instance's
+// mass is the same in any stack): thread t adds the 16-byte vectors t,
+// t + kMassThreads, ... of the instance's cells in turn, thread t < n mod V
+// then the tail cell n - n mod V + t, and a fixed tree combines the threads.
+// 1024 threads and dword-aligned vector loads keep enough bytes in flight
+// for a few blocks to stream large instances.
+constexpr int kMassThreads = 1024;
+
+template <typename V, typename T>
+__device__ __forceinline__ V load_dwords(const T* p) {
+    V v;
+    __builtin_memcpy(&v, __builtin_assume_aligned(p, 4), sizeof(V));
+    return v;
+}
+
 template <typename T>
-__global__ __launch_bounds__(kT) void instance_mass_kernel(const T* __restrict__ u, const int64_t* __restrict__ cstart,
-                                                           const double* __restrict__ scale, double* __restrict__ out) {
-    __shared__ double s[kT];
+__global__ __launch_bounds__(kMassThreads) void instance_mass_kernel(const T* __restrict__ u,
+                                                                    const int64_t* __restrict__ cstart,
+                                                                    const double* __restrict__ scale,
+                                                                    double* __restrict__ out) {
+    constexpr int V = 16 / (int)sizeof(T);
+    typedef T VT __attribute__((ext_vector_type(V)));
+    __shared__ double s[kMassThreads];
     const int32_t i = blockIdx.x;
-    const int64_t c0 = cstart[i], c1 = cstart[i + 1];
+    const int64_t c0 = cstart[i], n = cstart[i + 1] - c0;
+    const int64_t nv = n / V;
+    const T* base = u + c0;
     double acc = 0.0;
-    for (int64_t c = c0 + threadIdx.x; c < c1; c += kT) acc += (double)u[c];
+#pragma unroll 4
+    for (int64_t k = threadIdx.x; k < nv; k += kMassThreads) {
+        const VT v = load_dwords<VT>(base + k * V);
+#pragma unroll
+        for (int j = 0; j < V; ++j) acc += (double)v[j];
+    }
+    if ((int64_t)threadIdx.x < n - nv * V) acc += (double)base[nv * V + threadIdx.x];
     s[threadIdx.x] = acc;
     __syncthreads();
-    for (int w = kT / 2; w > 0; w >>= 1) {
+    for (int w = kMassThreads / 2; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
         __syncthreads();
     }
@@ -58,11 +84,11 @@ int32_t stx_instance_mass(int32_t dtype, const void* u, const int64_t* cstart, i
                           double* out, hipStream_t stream) {
     if (ninst <= 0) return STX_OK;
     if (dtype == 4)
-        hipLaunchKernelGGL(instance_mass_kernel<float>, dim3(ninst), dim3(kT), 0, stream, (const float*)u, cstart, scale,
-                           out);
+        hipLaunchKernelGGL(instance_mass_kernel<float>, dim3(ninst), dim3(kMassThreads), 0, stream, (const float*)u,
+                           cstart, scale, out);
     else if (dtype == 8)
-        hipLaunchKernelGGL(instance_mass_kernel<double>, dim3(ninst), dim3(kT), 0, stream, (const double*)u, cstart,
-                           scale, out);
+        hipLaunchKernelGGL(instance_mass_kernel<double>, dim3(ninst), dim3(kMassThreads), 0, stream, (const double*)u,
+                           cstart, scale, out);
     else
         return stx_fail(STX_EINVAL, "instance_mass: dtype must be 4 or 8");
     return stx_launch_status("instance_mass");
