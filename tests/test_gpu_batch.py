@@ -79,14 +79,20 @@ def test_batch_equals_alone_and_graph_equals_eager(torch_mod):
         assert many.masses().cpu().numpy()[k] == alone.masses().cpu().numpy()[0]
 
 
-def test_batched_upwind_matches_single_instance(torch_mod):
+@pytest.mark.parametrize("dt_name", ["f32", "f64"])
+def test_batched_upwind_matches_single_instance(torch_mod, dt_name):
+    """Stacked instances of sizes 16..72 (instance boundaries fall inside
+    waves, so lanes of one wave read two instances' coefficients): the
+    segment and row-group kernels agree, and each instance equals its
+    single-instance step."""
     torch = torch_mod
     import subsetix_amd as sx
     from subsetix_amd.batch import BatchConfig, BatchedAMR
     from subsetix_amd.interval_stencil import upwind_values
 
+    dtype = torch.float32 if dt_name == "f32" else torch.float64
     insts = _instances()
-    eng = BatchedAMR(insts, config=BatchConfig(levels=1, use_graph=False))
+    eng = BatchedAMR(insts, dtype=dtype, config=BatchConfig(levels=1, use_graph=False))
     f = eng._fields(eng.cur)[0]
     out = torch.empty_like(f.values)
     from subsetix_amd.levels import upwind_batched
@@ -99,11 +105,11 @@ def test_batched_upwind_matches_single_instance(torch_mod):
     for k, (inst, n) in enumerate(zip(insts, eng.sizes)):
         n = int(n)
         s = sx.full_interval_set(n, n)
-        fi = sx.create_interval_field(s, 0.0, dtype=torch.float32)
+        fi = sx.create_interval_field(s, 0.0, dtype=dtype)
         fi.values.copy_(f.values[off:off + n * n])
         o = torch.empty_like(fi.values)
         a, b = inst["velocity"]
-        dt = float(eng.inst_coef[0][5 * k + 2])  # the float32 coefficient the batched kernel uses
+        dt = float(eng.inst_coef[0][5 * k + 2])  # the coefficient (of the field's type) the batched kernel uses
         upwind_values(fi, fi.values, o, width=n, height=n, a=a, b=b, dt=dt, dx=1.0 / n, dy=1.0 / n)
         np.testing.assert_array_equal(o.cpu().numpy(), out[off:off + n * n].cpu().numpy())
         off += n * n
