@@ -262,14 +262,20 @@ __device__ __forceinline__ void instance_coef(const T* __restrict__ inst_coef, i
     }
 }
 
-// Cells [c, c + G) crossing a segment end or the field end: a per-cell walk
-// with scalar gathers.
+// Cells [c, c + G) crossing a segment end or the field end: a per-cell walk.
+// The own values and the cells beside the group come from cv / lv / rv (a
+// cell's left and right neighbours, when in the set, are the cells before
+// and after it in storage); the rows below and above are gathered per cell.
+// k0: coefficients of segment s's instance, read again only for a cell of
+// another instance.
 template <typename T, bool BATCH, class SG>
 __device__ __forceinline__ void upwind_cells(const SG& sg, int32_t s, int32_t c, int32_t C,
-                                             __amdgpu_buffer_rsrc_t rs, const T* __restrict__ u, T* __restrict__ out,
-                                             T a, T b, T dt, T dx, T dy, const T* __restrict__ inst_coef) {
+                                             __amdgpu_buffer_rsrc_t rs, T* __restrict__ out,
+                                             const typename Vec<T>::type& cv, T lv, T rv, const T* k0,
+                                             const T* __restrict__ inst_coef) {
     constexpr int G = Vec<T>::n;
     Seg g = sg.get(s);
+    const int32_t inst0 = g.inst;
 #pragma unroll
     for (int v = 0; v < G; ++v) {
         const int32_t cc = c + v;
@@ -278,24 +284,22 @@ __device__ __forceinline__ void upwind_cells(const SG& sg, int32_t s, int32_t c,
                 ++s;
                 g = sg.get(s);
             }
-            T ca = a, cb = b, cdt = dt, cdx = dx, cdy = dy;
-            if (BATCH) {
+            T k[5];
+#pragma unroll
+            for (int j = 0; j < 5; ++j) k[j] = k0[j];
+            if (BATCH && g.inst != inst0) {
                 const T* q = inst_coef + 5 * (int64_t)g.inst;
-                ca = q[0];
-                cb = q[1];
-                cdt = q[2];
-                cdx = q[3];
-                cdy = q[4];
-                if (cdx == (T)0) continue;  // retired instance
+#pragma unroll
+                for (int j = 0; j < 5; ++j) k[j] = q[j];
             }
+            if (BATCH && k[3] == (T)0) continue;  // retired instance
             const bool lok = (cc == g.c0) ? (g.flags & 1) : (g.flags & 2);
             const bool rok = (cc + 1 == g.cend) ? (g.flags & 4) : (g.flags & 2);
-            const T center = u[cc];
-            const T left = lok ? u[cc - 1] : (T)0;
-            const T right = rok ? u[cc + 1] : (T)0;
+            const T left = lok ? (v == 0 ? lv : cv[v - 1]) : (T)0;
+            const T right = rok ? (v == G - 1 ? rv : cv[v + 1]) : (T)0;
             const T down = gather<T>(rs, g.dd != kNone ? cc + g.dd : -1);
             const T up = gather<T>(rs, g.ud != kNone ? cc + g.ud : -1);
-            out[cc] = upwind_cell<T>(center, left, right, down, up, ca, cb, cdt, cdx, cdy);
+            out[cc] = upwind_cell<T>(cv[v], left, right, down, up, k[0], k[1], k[2], k[3], k[4]);
         }
     }
 }
@@ -347,10 +351,10 @@ struct UpwindGroup {
 
     template <bool BATCH, class SG>
     __device__ __forceinline__ void finish(const SG& sg, int32_t c, int32_t C, __amdgpu_buffer_rsrc_t rs,
-                                           const T* __restrict__ u, T* __restrict__ out, const VT& cv, T lv, T rv,
-                                           T a, T b, T dt, T dx, T dy, const T* __restrict__ inst_coef) const {
+                                           T* __restrict__ out, const VT& cv, T lv, T rv,
+                                           const T* __restrict__ inst_coef) const {
         if (!(bits & 1u)) {
-            upwind_cells<T, BATCH>(sg, s, c, C, rs, u, out, a, b, dt, dx, dy, inst_coef);
+            upwind_cells<T, BATCH>(sg, s, c, C, rs, out, cv, lv, rv, k, inst_coef);
             return;
         }
         const bool inner = bits & 8u;
@@ -393,8 +397,7 @@ __device__ __forceinline__ void upwind_thread(const SG& sg, int32_t s0, int32_t 
         for (int q = 0; q < NG; ++q) {
             const int32_t c = ct + (q * TH + (int32_t)threadIdx.x) * G;
             if (c < C)
-                grp[q].template finish<BATCH>(sg, c, C, rs, u, out, cv[q], lv[q], rv[q], a, b, dt, dx, dy,
-                                              inst_coef);
+                grp[q].template finish<BATCH>(sg, c, C, rs, out, cv[q], lv[q], rv[q], inst_coef);
         }
     } else {
 #pragma unroll
@@ -403,7 +406,7 @@ __device__ __forceinline__ void upwind_thread(const SG& sg, int32_t s0, int32_t 
             if (c < C) {
                 UpwindGroup<T> grp;
                 grp.template issue<BATCH>(sg, s0, s1, c, C, rs, a, b, dt, dx, dy, inst_coef);
-                grp.template finish<BATCH>(sg, c, C, rs, u, out, cv[q], lv[q], rv[q], a, b, dt, dx, dy, inst_coef);
+                grp.template finish<BATCH>(sg, c, C, rs, out, cv[q], lv[q], rv[q], inst_coef);
             }
         }
     }
@@ -547,12 +550,11 @@ struct TagGroup {
                 outside |= !(g.flags & 2);
                 const bool lok = (cc == g.c0) ? (g.flags & 1) : (g.flags & 2);
                 const bool rok = (cc + 1 == g.cend) ? (g.flags & 4) : (g.flags & 2);
-                const T center = u[cc];
-                const T left = gather<T>(rs, cc - 1);
-                const T right = gather<T>(rs, cc + 1);
+                const T left = v == 0 ? lv : cv[v - 1];
+                const T right = v == G - 1 ? rv : cv[v + 1];
                 const T down = gather<T>(rs, g.dd != kNone ? cc + g.dd : -1);
                 const T up = gather<T>(rs, g.ud != kNone ? cc + g.ud : -1);
-                if (grad_tag_cell<T>(center, left, right, down, up, lok, rok, g.dd != kNone, g.ud != kNone, thr))
+                if (grad_tag_cell<T>(cv[v], left, right, down, up, lok, rok, g.dd != kNone, g.ud != kNone, thr))
                     tags |= 1u << v;
                 if (lok) lex |= 1u << v;
             }
