@@ -56,10 +56,13 @@ constexpr int kXcds = 8;
 // Block -> tile order: the XCDs take chunks of kOrderChunk consecutive tiles
 // in turn (block b runs on XCD b % 8), so the rows above and below a tile are
 // mostly read through the same L2 while the whole chip sweeps the field front
-// to back (measured: same FP32 step time, 10 % faster FP64 steps than one
-// contiguous eighth per XCD).  Grids are padded to a multiple of
-// kOrderSpan, which makes the order a permutation of the padded grid.
-constexpr int64_t kOrderChunk = 8;
+// to back.  Measured on the composite step (4096^2 coarse, 8192^2 fine
+// discs): chunks of 32 tiles take 109.7 us (FP32) and 182 us (FP64) per
+// step, chunks of 8 112.8 and 195.5 us, chunks of 64 or 128 111.4 and
+// 184 us, chunks of 2 121.6 and 210.8 us; one contiguous eighth per XCD was
+// slower still for FP64.  Grids are padded to a multiple of kOrderSpan,
+// which makes the order a permutation of the padded grid.
+constexpr int64_t kOrderChunk = 32;
 constexpr int64_t kOrderSpan = kOrderChunk * kXcds;
 __device__ __forceinline__ int64_t xcd_chunk_order(int64_t block) {
     const int64_t xcd = block % kXcds, nth = block / kXcds;
