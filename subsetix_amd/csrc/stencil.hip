@@ -33,6 +33,9 @@
 // [0, height), so adjacency of row ids decides the vertical neighbours.
 #include <limits.h>
 
+#include <cmath>
+#include <limits>
+
 #include "stx_common.h"
 #include "stx_host.h"
 
@@ -470,9 +473,12 @@ __global__ __launch_bounds__(seg_threads<T>()) void upwind_seg_kernel(const int3
 //   3. tag_write: the r-th start / end of the field in storage order is the
 //      r-th run's begin / end (x from the cell's segment: x0 + c - c0), and
 //      the row offsets are the start counts before each row's first cell.
+// Tag test |grad u| >= t as gx^2 + gy^2 >= smin, smin = sqrt_threshold(t):
+// a correctly rounded square root is monotone, so sqrt(s) >= t exactly when
+// s >= smin (NaN s and NaN t fail both forms).
 template <typename T>
 __device__ __forceinline__ bool grad_tag_cell(T center, T left, T right, T down, T up, bool hl, bool hr, bool hd,
-                                              bool hu, T t) {
+                                              bool hu, T smin) {
     T gx = (T)0, gy = (T)0;
     if (hl && hr) gx = (right - left) * (T)0.5;
     else if (hr) gx = right - center;
@@ -482,7 +488,28 @@ __device__ __forceinline__ bool grad_tag_cell(T center, T left, T right, T down,
     else if (hd) gy = center - down;
     const T gx2 = gx * gx;
     const T gy2 = gy * gy;
-    return sqrt(gx2 + gy2) >= t;
+    return gx2 + gy2 >= smin;
+}
+
+// Smallest s >= 0 whose correctly rounded square root is >= t (host side,
+// binary search over the non-negative bit patterns); NaN for a NaN t.
+template <typename T, typename Bits>
+T sqrt_threshold(T t) {
+    if (t != t) return t;
+    if (!(t > (T)0)) return (T)0;
+    Bits lo = 0, hi;
+    const T inf = std::numeric_limits<T>::infinity();
+    __builtin_memcpy(&hi, &inf, sizeof(T));
+    while (lo < hi) {
+        const Bits mid = lo + (hi - lo) / 2;
+        T v;
+        __builtin_memcpy(&v, &mid, sizeof(T));
+        if (std::sqrt(v) >= t) hi = mid;
+        else lo = mid + 1;
+    }
+    T r;
+    __builtin_memcpy(&r, &lo, sizeof(T));
+    return r;
 }
 
 // Tag and left-neighbour bits of one group [c, c + G) (bit v = cell c + v),
@@ -521,7 +548,7 @@ struct TagGroup {
 
     template <class SG>
     __device__ __forceinline__ void finish(const SG& sg, int32_t c, int32_t C, __amdgpu_buffer_rsrc_t rs,
-                                           const T* __restrict__ u, const VT& cv, T lv, T rv, T thr, uint32_t& tags,
+                                           const T* __restrict__ u, const VT& cv, T lv, T rv, T smin, uint32_t& tags,
                                            uint32_t& lex, bool& outside) const {
         tags = 0u;
         lex = 0u;
@@ -535,7 +562,7 @@ struct TagGroup {
                 const bool rok = v == G - 1 ? (bool)(bits & 4u) : inner;
                 const T left = v == 0 ? lv : cv[v - 1];
                 const T right = v == G - 1 ? rv : cv[v + 1];
-                if (grad_tag_cell<T>(cv[v], left, right, dv[v], uv[v], lok, rok, hd, hu, thr)) tags |= 1u << v;
+                if (grad_tag_cell<T>(cv[v], left, right, dv[v], uv[v], lok, rok, hd, hu, smin)) tags |= 1u << v;
                 if (lok) lex |= 1u << v;
             }
             return;
@@ -557,7 +584,7 @@ struct TagGroup {
                 const T right = v == G - 1 ? rv : cv[v + 1];
                 const T down = gather<T>(rs, g.dd != kNone ? cc + g.dd : -1);
                 const T up = gather<T>(rs, g.ud != kNone ? cc + g.ud : -1);
-                if (grad_tag_cell<T>(cv[v], left, right, down, up, lok, rok, g.dd != kNone, g.ud != kNone, thr))
+                if (grad_tag_cell<T>(cv[v], left, right, down, up, lok, rok, g.dd != kNone, g.ud != kNone, smin))
                     tags |= 1u << v;
                 if (lok) lex |= 1u << v;
             }
@@ -572,7 +599,7 @@ struct TagGroup {
 // loads before finishing either; FP64 (four groups) one group at a time.
 template <typename T, class SG>
 __device__ __forceinline__ void tag_thread(const SG& sg, int32_t s0, int32_t s1, int32_t ct, int32_t C,
-                                           __amdgpu_buffer_rsrc_t rs, const T* __restrict__ u, T thr,
+                                           __amdgpu_buffer_rsrc_t rs, const T* __restrict__ u, T smin,
                                            const typename Vec<T>::type* cv, const T* lv, const T* rv,
                                            uint32_t* __restrict__ tag_w, uint32_t* __restrict__ lex_w, int64_t W,
                                            bool& outside) {
@@ -602,7 +629,7 @@ __device__ __forceinline__ void tag_thread(const SG& sg, int32_t s0, int32_t s1,
         for (int q = 0; q < NG; ++q) {
             const int32_t c = ct + (q * kThreads + (int32_t)threadIdx.x) * G;
             uint32_t tg = 0u, lx = 0u;
-            if (c < C) grp[q].finish(sg, c, C, rs, u, cv[q], lv[q], rv[q], thr, tg, lx, outside);
+            if (c < C) grp[q].finish(sg, c, C, rs, u, cv[q], lv[q], rv[q], smin, tg, lx, outside);
             word(c, tg, lx);
         }
     } else {
@@ -613,7 +640,7 @@ __device__ __forceinline__ void tag_thread(const SG& sg, int32_t s0, int32_t s1,
             if (c < C) {
                 TagGroup<T> grp;
                 grp.issue(sg, s0, s1, c, C, rs);
-                grp.finish(sg, c, C, rs, u, cv[q], lv[q], rv[q], thr, tg, lx, outside);
+                grp.finish(sg, c, C, rs, u, cv[q], lv[q], rv[q], smin, tg, lx, outside);
             }
             word(c, tg, lx);
         }
@@ -621,11 +648,12 @@ __device__ __forceinline__ void tag_thread(const SG& sg, int32_t s0, int32_t s1,
 }
 
 // Pass 1.  Own values and the cells beside each group load first (they do
-// not depend on the segments); the tile's segments then go to LDS.
+// not depend on the segments); the tile's segments then go to LDS.  smin:
+// sqrt_threshold of the tag threshold.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void tag_bits_kernel(const int32_t* __restrict__ plan, int64_t ntiles,
                                                             int32_t C, const int32_t* __restrict__ segs,
-                                                            const T* __restrict__ u, T thr,
+                                                            const T* __restrict__ u, T smin,
                                                             uint32_t* __restrict__ tag_w, uint32_t* __restrict__ lex_w,
                                                             int64_t W, uint32_t* __restrict__ status) {
     constexpr int G = Vec<T>::n;
@@ -651,9 +679,9 @@ __global__ __launch_bounds__(kThreads) void tag_bits_kernel(const int32_t* __res
     if (M <= kSegCap) {
         for (int q = threadIdx.x; q < 2 * M; q += kThreads) s_seg[q] = gs[2 * (int64_t)s0 + q];
         __syncthreads();
-        tag_thread<T>(LdsSegs{s_seg, s0}, s0, s1, ct, C, rs, u, thr, cv, lv, rv, tag_w, lex_w, W, outside);
+        tag_thread<T>(LdsSegs{s_seg, s0}, s0, s1, ct, C, rs, u, smin, cv, lv, rv, tag_w, lex_w, W, outside);
     } else {
-        tag_thread<T>(GlobalSegs{gs}, s0, s1, ct, C, rs, u, thr, cv, lv, rv, tag_w, lex_w, W, outside);
+        tag_thread<T>(GlobalSegs{gs}, s0, s1, ct, C, rs, u, smin, cv, lv, rv, tag_w, lex_w, W, outside);
     }
     if (outside) atomicOr(status, STX_FLAG_DOMAIN);
 }
@@ -838,10 +866,12 @@ int32_t stx_grad_tag_seg(int32_t dtype, const int32_t* plan, int64_t ntiles, int
         const int64_t grid = (ntiles + kOrderSpan - 1) / kOrderSpan * kOrderSpan;
         if (dtype == 4)
             hipLaunchKernelGGL(tag_bits_kernel<float>, dim3((unsigned)grid), dim3(kThreads), 0, stream, plan, ntiles,
-                               (int32_t)C, segs, (const float*)u, (float)t, tag_w, lex_w, W, status);
+                               (int32_t)C, segs, (const float*)u, sqrt_threshold<float, uint32_t>((float)t), tag_w,
+                               lex_w, W, status);
         else if (dtype == 8)
             hipLaunchKernelGGL(tag_bits_kernel<double>, dim3((unsigned)grid), dim3(kThreads), 0, stream, plan, ntiles,
-                               (int32_t)C, segs, (const double*)u, t, tag_w, lex_w, W, status);
+                               (int32_t)C, segs, (const double*)u, sqrt_threshold<double, uint64_t>(t), tag_w, lex_w,
+                               W, status);
         else
             return stx_fail(STX_EINVAL, "grad_tag_seg: dtype must be 4 or 8");
         hipLaunchKernelGGL(tag_runs_kernel, dim3(nblocks(W)), dim3(256), 0, stream, tag_w, lex_w, W, st_w, en_w, cnt);
