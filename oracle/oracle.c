@@ -797,6 +797,167 @@ int ox_subset_copy(int mode, const ox_set* sup, void* sup_vals, const ox_set* su
     return OX_OK;
 }
 
+
+/* ---------------------------------------------------------------------- */
+/* Level transfers onto a target subset, by cell lookup.  The reference's
+ * synchronisation (fields.py:570-612) restricts the whole fine field with
+ * restrict_field (multilevel.py:410-479), gathers the refine cells and
+ * scatters them into the coarse field, then gathers the coarse-only cells,
+ * prolongs them (multilevel.py:365-407) and scatters them into the fine
+ * field.  Cell for cell that is:
+ *   prolong onto T:  dst(y, x) = src(floor(y/R), floor(x/R))
+ *   restrict onto T: dst(y, x) = (sum over dy < R, then dx < R, of
+ *                    src(yR+dy, xR+dx)) * (1/R^2)
+ * for every cell (y, x) of T.  This restatement looks every cell up by row
+ * (binary search over the row ids) and interval (binary search in the row),
+ * with no segment tables and no cursors, so it shares no indexing with the
+ * HIP kernels.  Cells of T missing from dst are skipped; a missing prolong
+ * parent leaves dst as it is; missing restrict children add nothing.
+ * kind: 0 = float values, float sums, norm = (float)(1/R^2) (the reference
+ * FP32 kernel, multilevel.py:462-479); 1 = float values, double sums, then
+ * (float)(sum * (1/R^2)); 2 = double throughout.  mode 0 = prolong,
+ * 1 = restrict; ratio 1 prolong is a plain copy of the common cells. */
+typedef struct {
+    const ox_set* s;
+    int32_t* co;
+} ox_cells;
+
+static void cells_init(ox_cells* c, const ox_set* s) {
+    c->s = s;
+    c->co = cell_offsets(s);
+}
+
+static int64_t cell_find(const ox_cells* c, int32_t y, int32_t x) {
+    const ox_set* s = c->s;
+    int64_t r = lower_bound_i32(s->rows, s->nrows, y);
+    if (r >= s->nrows || s->rows[r] != y) return -1;
+    int32_t lo = s->off[r], hi = s->off[r + 1] - 1;
+    while (lo <= hi) {
+        int32_t mid = lo + ((hi - lo) >> 1);
+        if (x < s->beg[mid]) hi = mid - 1;
+        else if (x >= s->end[mid]) lo = mid + 1;
+        else return (int64_t)c->co[mid] + (x - s->beg[mid]);
+    }
+    return -1;
+}
+
+int ox_transfer(int mode, int kind, int32_t ratio, const ox_set* t, const ox_set* d, void* dvals, const ox_set* s,
+                const void* svals) {
+    if (ratio < 1 || ratio > 64 || (mode != 0 && mode != 1) || kind < 0 || kind > 2) return OX_EINVAL;
+    ox_cells dc, sc;
+    cells_init(&dc, d);
+    cells_init(&sc, s);
+    int32_t* tir = interval_rows(t);
+    const float normf = (float)(1.0 / (double)(ratio * ratio));
+    const double normd = 1.0 / (double)(ratio * ratio);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t i = 0; i < t->n; ++i) {
+        const int32_t y = tir[i];
+        for (int32_t x = t->beg[i]; x < t->end[i]; ++x) {
+            const int64_t di = cell_find(&dc, y, x);
+            if (di < 0) continue;
+            if (mode == 0) {
+                const int64_t si = cell_find(&sc, floordiv(y, ratio), floordiv(x, ratio));
+                if (si < 0) continue;
+                if (kind == 2) ((double*)dvals)[di] = ((const double*)svals)[si];
+                else ((float*)dvals)[di] = ((const float*)svals)[si];
+                continue;
+            }
+            float accf = 0.0f;
+            double accd = 0.0;
+            for (int32_t dy = 0; dy < ratio; ++dy)
+                for (int32_t dx = 0; dx < ratio; ++dx) {
+                    const int64_t si = cell_find(&sc, y * ratio + dy, x * ratio + dx);
+                    if (si < 0) continue;
+                    if (kind == 0) accf = accf + ((const float*)svals)[si];
+                    else if (kind == 1) accd = accd + (double)((const float*)svals)[si];
+                    else accd = accd + ((const double*)svals)[si];
+                }
+            if (kind == 0) ((float*)dvals)[di] = accf * normf;
+            else if (kind == 1) ((float*)dvals)[di] = (float)(accd * normd);
+            else ((double*)dvals)[di] = accd * normd;
+        }
+    }
+    free(tir); free(dc.co); free(sc.co);
+    return OX_OK;
+}
+
+/* Gradient tag on a sparse level: the central-difference gradient of
+ * regrid.py:181-218, restated for an interval field whose neighbours may be
+ * absent (a neighbour is a cell of the same set; one-sided differences where
+ * only one exists, 0 where neither does), then maximal runs of
+ * sqrt(gx^2 + gy^2) >= t per row (regrid.py:20-124), the rows being the
+ * set's rows.  esize 4: float arithmetic, t rounded to float; 8: double.
+ * Runs are maximal over adjacent cells, also across touching intervals. */
+#define GRAD_TAG_BODY(T, SQRT)                                                                       \
+    const T* u = (const T*)vals;                                                                     \
+    const T tt = (T)t;                                                                               \
+    unsigned char* tag = (unsigned char*)xalloc((size_t)(nc > 0 ? nc : 1));                          \
+    _Pragma("omp parallel for schedule(dynamic, 64)")                                                \
+    for (int64_t i = 0; i < s->n; ++i) {                                                             \
+        const int32_t y = ir[i];                                                                     \
+        for (int32_t x = s->beg[i]; x < s->end[i]; ++x) {                                            \
+            const int64_t c = (int64_t)cs.co[i] + (x - s->beg[i]);                                   \
+            const int64_t l = cell_find(&cs, y, x - 1), r = cell_find(&cs, y, x + 1);                \
+            const int64_t dn = cell_find(&cs, y - 1, x), up = cell_find(&cs, y + 1, x);              \
+            T gx = (T)0, gy = (T)0;                                                                  \
+            if (l >= 0 && r >= 0) gx = (u[r] - u[l]) * (T)0.5;                                       \
+            else if (r >= 0) gx = u[r] - u[c];                                                       \
+            else if (l >= 0) gx = u[c] - u[l];                                                       \
+            if (dn >= 0 && up >= 0) gy = (u[up] - u[dn]) * (T)0.5;                                   \
+            else if (up >= 0) gy = u[up] - u[c];                                                     \
+            else if (dn >= 0) gy = u[c] - u[dn];                                                     \
+            const T gx2 = gx * gx;                                                                   \
+            const T gy2 = gy * gy;                                                                   \
+            tag[c] = SQRT(gx2 + gy2) >= tt;                                                          \
+        }                                                                                            \
+    }
+
+int ox_grad_tag(const ox_set* s, const void* vals, int esize, double t, ox_set* out) {
+    if (esize != 4 && esize != 8) return OX_EINVAL;
+    ox_cells cs;
+    cells_init(&cs, s);
+    const int64_t nc = cs.co[s->n];
+    int32_t* ir = interval_rows(s);
+    unsigned char* tagp;
+    {
+        if (esize == 4) {
+            GRAD_TAG_BODY(float, sqrtf)
+            tagp = tag;
+        } else {
+            GRAD_TAG_BODY(double, sqrt)
+            tagp = tag;
+        }
+    }
+    /* runs: a tagged cell starts a run unless the previous cell of the row
+     * (x - 1, in the set) is tagged */
+    int32_t* cnt = (int32_t*)xalloc(sizeof(int32_t) * (size_t)(s->nrows > 0 ? s->nrows : 1));
+    int64_t total = 0;
+    for (int pass = 0; pass < 2; ++pass) {
+        if (pass == 1) {
+            set_alloc(out, s->nrows, total);
+            for (int64_t r = 0; r < s->nrows; ++r) { out->rows[r] = s->rows[r]; out->off[r + 1] = out->off[r] + cnt[r]; }
+        }
+        for (int64_t r = 0; r < s->nrows; ++r) {
+            int32_t k = 0, prev_x = 0, wp = pass ? out->off[r] : 0;
+            int prev_on = 0;
+            for (int32_t i = s->off[r]; i < s->off[r + 1]; ++i)
+                for (int32_t x = s->beg[i]; x < s->end[i]; ++x) {
+                    const int on = tagp[cs.co[i] + (x - s->beg[i])];
+                    const int joined = prev_on && prev_x == x - 1;
+                    if (prev_on && !(on && joined)) { if (pass) out->end[wp++] = prev_x + 1; }
+                    if (on && !joined) { if (pass) out->beg[wp] = x; ++k; }
+                    prev_on = on;
+                    prev_x = x;
+                }
+            if (prev_on && pass) out->end[wp++] = prev_x + 1;
+            if (!pass) { cnt[r] = k; total += k; }
+        }
+    }
+    free(cnt); free(tagp); free(ir); free(cs.co);
+    return OX_OK;
+}
+
 int ox_num_threads(void) {
 #ifdef _OPENMP
     return omp_get_max_threads();

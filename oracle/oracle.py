@@ -388,6 +388,214 @@ def synchronize(coarse: OSet, cvals, fine: OSet, fvals, refine: OSet, *, ratio, 
     return cv, fv
 
 
+_KIND = {(np.dtype(np.float32), "f32"): 0, (np.dtype(np.float32), "f64"): 1}
+
+
+def _transfer(mode: int, target: OSet, dst: OSet, dvals: np.ndarray, src: OSet, svals: np.ndarray, ratio: int,
+              accumulate: str) -> None:
+    assert dvals.flags.c_contiguous and svals.flags.c_contiguous and dvals.dtype == svals.dtype
+    kind = 2 if dvals.dtype == np.float64 else _KIND[(dvals.dtype, accumulate)]
+    _check(lib().ox_transfer(ctypes.c_int(mode), ctypes.c_int(kind), ctypes.c_int32(ratio),
+                             ctypes.byref(_to_c(target)), ctypes.byref(_to_c(dst)),
+                             dvals.ctypes.data_as(ctypes.c_void_p), ctypes.byref(_to_c(src)),
+                             svals.ctypes.data_as(ctypes.c_void_p)))
+
+
+def prolong_onto(target: OSet, dst: OSet, dvals: np.ndarray, src: OSet, svals: np.ndarray, ratio: int) -> None:
+    """dst(y, x) = src(floor(y/R), floor(x/R)) on the cells of ``target``, in
+    place: the prolong + scatter half of fields.py:570-612 (multilevel.py:
+    365-407) cell by cell."""
+    _transfer(0, target, dst, dvals, src, svals, ratio, "f32")
+
+
+def restrict_onto(target: OSet, dst: OSet, dvals: np.ndarray, src: OSet, svals: np.ndarray, ratio: int,
+                  accumulate: str = "f64") -> None:
+    """dst(y, x) = mean of the children src(yR+dy, xR+dx) (dy-major, missing
+    children add nothing) on the cells of ``target``, in place: the restrict +
+    gather + scatter half of fields.py:570-612 (multilevel.py:410-479)."""
+    _transfer(1, target, dst, dvals, src, svals, ratio, accumulate)
+
+
+def grad_tag(s: OSet, vals: np.ndarray, t: float) -> OSet:
+    """Gradient tag on a sparse level (regrid.py:181-218 differences with
+    one-sided forms where a neighbour is missing; regrid.py:20-124 runs)."""
+    v = np.ascontiguousarray(vals)
+    out = _CSet()
+    _check(lib().ox_grad_tag(ctypes.byref(_to_c(s)), v.ctypes.data_as(ctypes.c_void_p), ctypes.c_int(v.itemsize),
+                             ctypes.c_double(t), ctypes.byref(out)))
+    return _from_c(out)
+
+
+def zeros_on(s: OSet, dtype) -> np.ndarray:
+    return np.zeros(int(cell_offsets(s)[-1]), dtype=dtype)
+
+
+class Composite:
+    """Sparse two-level composite step: the reference's step (simulation.py:
+    232-354: synchronise, advance both levels, synchronise) on a full coarse
+    level and a fine level that covers prolong(refine) plus a one-cell ghost
+    ring (morphology.py:322-341).  One step:
+      1. prolong coarse -> fine ghost cells;
+      2. upwind on the coarse level and on the extended fine level
+         (interval_stencil.py:16-219);
+      3. restrict fine -> coarse on the refined cells (sums in ``accumulate``).
+    """
+
+    def __init__(self, width: int, height: int, refine: OSet, *, velocity=(1.0, 0.5), cfl=0.5, dtype=np.float32,
+                 accumulate="f64", ratio=2):
+        self.W, self.H, self.R = int(width), int(height), int(ratio)
+        self.coarse = full(self.W, self.H)
+        self.refine = clip(refine, self.W, self.H)
+        self.fine = prolong_set(self.refine, self.R)
+        fw, fh = self.W * self.R, self.H * self.R
+        self.ghost = ghost_zones(self.fine, 1, 1, width=fw, height=fh)
+        self.ext = union(self.fine, self.ghost)
+        self.dtype = np.dtype(dtype)
+        self.accumulate = accumulate
+        self.vel = (float(velocity[0]), float(velocity[1]))
+        self.dx_c, self.dy_c = 1.0 / self.W, 1.0 / self.H
+        self.dx_f, self.dy_f = self.dx_c / self.R, self.dy_c / self.R
+        denom = abs(self.vel[0]) / self.dx_f + abs(self.vel[1]) / self.dy_f
+        self.dt = cfl / denom if denom > 0 else 0.0
+        self.cv = zeros_on(self.coarse, self.dtype)
+        self.fv = zeros_on(self.ext, self.dtype)
+
+    def set_values(self, cv, fv=None) -> None:
+        """As the device driver: coarse values, fine values on the extended
+        set (default prolonged), then restrict and refresh the ghosts."""
+        self.cv = np.ascontiguousarray(cv, dtype=self.dtype).copy()
+        if fv is None:
+            self.fv = zeros_on(self.ext, self.dtype)
+            prolong_onto(self.ext, self.ext, self.fv, self.coarse, self.cv, self.R)
+        else:
+            self.fv = np.ascontiguousarray(fv, dtype=self.dtype).copy()
+        restrict_onto(self.refine, self.coarse, self.cv, self.ext, self.fv, self.R, self.accumulate)
+        prolong_onto(self.ghost, self.ext, self.fv, self.coarse, self.cv, self.R)
+
+    def step(self) -> None:
+        a, b = self.vel
+        prolong_onto(self.ghost, self.ext, self.fv, self.coarse, self.cv, self.R)
+        nc = upwind(self.coarse, self.cv, width=self.W, height=self.H, a=a, b=b, dt=self.dt, dx=self.dx_c,
+                    dy=self.dy_c)
+        nf = upwind(self.ext, self.fv, width=self.W * self.R, height=self.H * self.R, a=a, b=b, dt=self.dt,
+                    dx=self.dx_f, dy=self.dy_f)
+        restrict_onto(self.refine, self.coarse, nc, self.ext, nf, self.R, self.accumulate)
+        self.cv, self.fv = nc, nf
+
+    def mass(self) -> float:
+        return mass(self.cv, self.dx_c * self.dy_c)
+
+
+class Instance3:
+    """One instance of the batched three-level advection engine, restated on
+    oracle primitives with the instance's rows starting at 0.
+
+    The step follows the reference's step semantics (simulation.py:232-354)
+    extended to three levels: prolong ghost cells level by level (coarse to
+    fine), upwind every level (interval_stencil.py:16-219, level L with
+    dx = dy = 1/(n 2^L)), restrict finest to coarsest onto the refine sets.
+    The regrid is the gradient tag (regrid.py:336-350 on the level's cells),
+    the grading dilation (regrid.py:404-442, square element), a one-cell
+    level-1 ring forced onto its parents and the level-2 tag gated by the
+    eroded level 1: the three-level nesting rules of test_amr_grading.py:
+    110-248 (level 2 inside level 1, no 8-neighbour contact between level 2
+    and cells outside level 1).  New level fields are prolonged from the
+    coarser level, then keep the previous values where the level existed.
+    """
+
+    def __init__(self, n: int, u0: np.ndarray, velocity, *, dtype=np.float32, levels=3, ratio=2, cfl=0.5,
+                 threshold=(0.02, 0.02), grading=1, regrid_every=8, accumulate="f64"):
+        self.n, self.L, self.R = int(n), int(levels), int(ratio)
+        self.dtype = np.dtype(dtype)
+        self.thr = tuple(threshold)
+        self.p = int(grading)
+        self.k = int(regrid_every)
+        self.accumulate = accumulate
+        self.D = [full(self.n * self.R ** lv, self.n * self.R ** lv) for lv in range(self.L)]
+        finest = self.n * self.R ** (self.L - 1)
+        denom = abs(velocity[0]) * finest + abs(velocity[1]) * finest
+        dt = cfl / max(denom, 1e-300) if denom > 0 else 0.0
+        # coefficients as the device stores them: rounded to the field type
+        self.coef = [tuple(float(v) for v in np.array([velocity[0], velocity[1], dt, 1.0 / (self.n * self.R ** lv),
+                                                       1.0 / (self.n * self.R ** lv)]).astype(self.dtype))
+                     for lv in range(self.L)]
+        self.ext = [self.D[0]]
+        self.vals = [np.ascontiguousarray(u0, dtype=self.dtype).reshape(-1).copy()]
+        self.sets = None
+        self.step_count = 0
+        self.regrid()
+        self.last_regrid = 0
+
+    def _tag(self, s: OSet, v: np.ndarray, lv: int) -> OSet:
+        return grad_tag(s, v, self.thr[min(lv, len(self.thr) - 1)])
+
+    def _level_field(self, lv, target, coarser_set, coarser_vals, old_ext, old_vals, old_sets):
+        f = zeros_on(target, self.dtype)
+        prolong_onto(target, target, f, coarser_set, coarser_vals, self.R)
+        if old_sets is not None and lv < len(old_vals) and lv < len(old_sets):
+            keep = intersection(target, old_sets[lv])
+            prolong_onto(keep, target, f, old_ext[lv], old_vals[lv], 1)
+        return f
+
+    def regrid(self) -> None:
+        R, p, L, D = self.R, self.p, self.L, self.D
+        old_ext, old_vals, old_sets = list(self.ext), list(self.vals), self.sets
+        cur0 = self.vals[0]
+        tag0 = self._tag(D[0], cur0, 0)
+        refine0 = intersection(dilate(tag0, p, p) if p > 0 else tag0, D[0])
+        sets, refines = [D[0]], []
+        if L >= 2:
+            ring = intersection(dilate(prolong_set(refine0, R), 1, 1), D[1])
+            refine0 = union(refine0, restrict_set(ring, R))
+            l1 = prolong_set(refine0, R)
+            sets.append(l1)
+            refines.append(refine0)
+        if L >= 3:
+            tmp1 = self._level_field(1, l1, D[0], cur0, old_ext, old_vals, old_sets)
+            comp = difference(D[1], l1)
+            eroded = difference(l1, dilate(comp, 1, 1))
+            tag1 = self._tag(l1, tmp1, 1)
+            refine1 = intersection(dilate(tag1, p, p) if p > 0 else tag1, eroded)
+            sets.append(prolong_set(refine1, R))
+            refines.append(refine1)
+        ghosts, ext, vals = [None], [D[0]], [cur0]
+        for lv in range(1, L):
+            g = difference(intersection(dilate(sets[lv], 1, 1), D[lv]), sets[lv])
+            e = union(sets[lv], g)
+            ghosts.append(g)
+            ext.append(e)
+            vals.append(self._level_field(lv, e, ext[lv - 1], vals[lv - 1], old_ext, old_vals, old_sets))
+        self.sets, self.refines, self.ghosts, self.ext, self.vals = sets, refines, ghosts, ext, vals
+
+    def step(self) -> None:
+        R, L = self.R, self.L
+        for lv in range(1, L):
+            prolong_onto(self.ghosts[lv], self.ext[lv], self.vals[lv], self.ext[lv - 1], self.vals[lv - 1], R)
+        nxt = []
+        for lv in range(L):
+            a, b, dt, dx, dy = self.coef[lv]
+            w = self.n * R ** lv
+            # the level sets may list empty rows just outside the instance
+            # (dilation row convention); they hold no cells
+            nxt.append(upwind(compact_rows(self.ext[lv]), self.vals[lv], width=w, height=w, a=a, b=b, dt=dt,
+                              dx=dx, dy=dy))
+        for lv in range(L - 1, 0, -1):
+            restrict_onto(self.refines[lv - 1], self.ext[lv - 1], nxt[lv - 1], self.ext[lv], nxt[lv], R,
+                          self.accumulate)
+        self.vals = nxt
+
+    def run(self, steps: int) -> None:
+        for _ in range(int(steps)):
+            if self.step_count % self.k == 0 and self.last_regrid != self.step_count:
+                self.regrid()
+                self.last_regrid = self.step_count
+            self.step()
+            self.step_count += 1
+
+    def mass(self) -> float:
+        return mass(self.vals[0], 1.0 / (self.n * self.n))
+
+
 def set_threads(n: int) -> None:
     lib().ox_set_threads(ctypes.c_int(n))
 

@@ -237,17 +237,17 @@ template <typename T>
 __device__ __forceinline__ T gather(__amdgpu_buffer_rsrc_t r, int32_t idx);
 template <>
 __device__ __forceinline__ float gather<float>(__amdgpu_buffer_rsrc_t r, int32_t idx) {
-    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, idx * 4, 0, 0));
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, buffer_byte_offset(idx, 4u), 0, 0));
 }
 template <>
 __device__ __forceinline__ double gather<double>(__amdgpu_buffer_rsrc_t r, int32_t idx) {
-    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, idx * 8, 0, 0));
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, buffer_byte_offset(idx, 8u), 0, 0));
 }
 
 template <typename T>
 __device__ __forceinline__ typename Vec<T>::type gather_vec(__amdgpu_buffer_rsrc_t r, int32_t idx) {
     return __builtin_bit_cast(typename Vec<T>::type,
-                              __builtin_amdgcn_raw_buffer_load_b128(r, idx * (int)sizeof(T), 0, 0));
+                              __builtin_amdgcn_raw_buffer_load_b128(r, buffer_byte_offset(idx, (uint32_t)sizeof(T)), 0, 0));
 }
 
 // Coefficients a, b, dt, dx, dy of instance `inst` of a stack.  The first

@@ -31,6 +31,16 @@ enum : uint32_t {
     STX_FLAG_DOMAIN = 16u,  // cells outside the [0, w) domain a segment table was built for
 };
 
+// Byte offset of element `idx` for a raw buffer load, computed in uint32 so
+// that fields of up to 4 GiB address every element without signed overflow.
+// A negative index (-1 for a missing neighbour, -vector width for a missing
+// row) becomes an offset within 64 bytes of 2^32, above the record size of
+// every buffer resource built here (< 0xffffff00 bytes), so the load returns
+// 0: the zero exterior of the stencils and the absent children of transfers.
+__device__ __forceinline__ int32_t buffer_byte_offset(int32_t idx, uint32_t elem_bytes) {
+    return (int32_t)((uint32_t)idx * elem_bytes);
+}
+
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (STX_WAVE - 1)); }
 __device__ __forceinline__ int wave_id() { return (int)(threadIdx.x / STX_WAVE); }
 

@@ -352,11 +352,11 @@ template <typename T>
 __device__ __forceinline__ T gatherT(__amdgpu_buffer_rsrc_t r, int32_t idx);
 template <>
 __device__ __forceinline__ float gatherT<float>(__amdgpu_buffer_rsrc_t r, int32_t idx) {
-    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, idx * 4, 0, 0));
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, buffer_byte_offset(idx, 4u), 0, 0));
 }
 template <>
 __device__ __forceinline__ double gatherT<double>(__amdgpu_buffer_rsrc_t r, int32_t idx) {
-    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, idx * 8, 0, 0));
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, buffer_byte_offset(idx, 8u), 0, 0));
 }
 
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc_of(const void* p, int64_t bytes) {

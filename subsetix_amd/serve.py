@@ -344,7 +344,7 @@ class QueryBatch:
                 o = off[k:k + nr + 1].copy()
                 one[name] = (o - o[0], ob[o[0]:o[-1]].copy(), oe[o[0]:o[-1]].copy())
             res.append(one)
-            k += nr + 16
+            k += nr  # the gap between queries is in the row ids only, not in the CSR rows
         self._keep = []
         return res
 

@@ -167,7 +167,8 @@ def test_sync_and_gather_golden(sx, golden):
         synchronize_interval_fields(coarse, fine, actions, ratio=3)
 
 
-def _oracle_amr(u0, n, steps, regrid_every, vel, cfl, thr, grading, mode, tag, accumulate, dtype):
+def _oracle_amr(u0, n, steps, regrid_every, vel, cfl, thr, grading, mode, tag, accumulate, dtype,
+                return_refine=False):
     """simulation.py:232-354 restated with oracle primitives (full-domain
     levels, ratio 2)."""
     w = h = n
@@ -191,13 +192,18 @@ def _oracle_amr(u0, n, steps, regrid_every, vel, cfl, thr, grading, mode, tag, a
         cv = ox.upwind(coarse, cv, width=w, height=h, a=vel[0], b=vel[1], dt=dt, dx=dxc, dy=dxc)
         fv = ox.upwind(fine, fv, width=2 * w, height=2 * h, a=vel[0], b=vel[1], dt=dt, dx=dxf, dy=dxf)
         cv, fv = ox.synchronize(coarse, cv, fine, fv, refine, ratio=2, accumulate=accumulate)
-    return cv, fv
+    return (cv, fv, refine) if return_refine else (cv, fv)
 
 
 @pytest.mark.parametrize("dtype", ["f32", "f64"])
-def test_amr_loop_vs_oracle(sx, dtype):
+@pytest.mark.parametrize("accumulate", ["f32", "f64"])
+@pytest.mark.parametrize("mode", ["von_neumann", "moore"])
+@pytest.mark.parametrize("tag", ["gradient", "value"])
+def test_amr_loop_vs_oracle(sx, dtype, accumulate, mode, tag):
     """The full step loop (sync, regrid, advance, sync) on a 32x32 Gaussian
-    bump, bit-exact against the same loop on oracle primitives."""
+    bump, bit-exact against the same loop on oracle primitives, for every
+    regrid criterion (the reference default: gradient tag, von Neumann
+    grading, FP32 sums) and option."""
     import torch
 
     from subsetix_amd.amr2 import AMR2Simulation, SimulationConfig
@@ -208,14 +214,17 @@ def test_amr_loop_vs_oracle(sx, dtype):
     u0 = np.exp(-((X - 0.3) ** 2 + (Y - 0.3) ** 2) / (2 * 0.1 ** 2))
     npdt = np.float32 if dtype == "f32" else np.float64
     tdt = torch.float32 if dtype == "f32" else torch.float64
-    cfg = SimulationConfig(coarse_resolution=n, velocity=(1.0, 0.5), cfl=0.5, refine_threshold=0.1, grading=2,
-                           grading_mode="moore", tag="value", accumulate="f64")
+    thr = 0.1 if tag == "value" else 0.02
+    cfg = SimulationConfig(coarse_resolution=n, velocity=(1.0, 0.5), cfl=0.5, refine_threshold=thr, grading=2,
+                           grading_mode=mode, tag=tag, accumulate=accumulate)
     sim = AMR2Simulation(cfg)
     field = sx.create_interval_field(sx.full_interval_set(n, n), 0.0, dtype=tdt)
     field.values.copy_(torch.from_numpy(u0.reshape(-1).astype(npdt)))
     sim.set_initial_field(field)
     sim.run(steps=12, regrid_every=5)
-    cv, fv = _oracle_amr(u0.reshape(-1), n, 12, 5, (1.0, 0.5), 0.5, 0.1, 2, "moore", "value", "f64", npdt)
+    cv, fv, refine = _oracle_amr(u0.reshape(-1), n, 12, 5, (1.0, 0.5), 0.5, thr, 2, mode, tag, accumulate, npdt,
+                                 return_refine=True)
+    assert 0 < refine.n
     np.testing.assert_array_equal(sim.state.coarse_field.values.cpu().numpy(), cv)
     np.testing.assert_array_equal(sim.state.fine_field.values.cpu().numpy(), fv)
     m = sim.total_mass()

@@ -281,3 +281,53 @@ def test_composite_steps_vs_oracle(sx, dt):
     np.testing.assert_array_equal(comp.coarse.values.cpu().numpy(), hc)
     np.testing.assert_array_equal(comp.fine_field.values.cpu().numpy(), hf)
     assert abs(comp.mass() - ox.mass(hc, 1.0 / W / H)) < 1e-12
+
+
+def _ulp_neighbours(t, npdt, k=3):
+    """t and its k nearest representable neighbours on each side."""
+    out = [npdt(t)]
+    lo = hi = npdt(t)
+    for _ in range(k):
+        lo = np.nextafter(lo, npdt(-np.inf))
+        hi = np.nextafter(hi, npdt(np.inf))
+        out += [lo, hi]
+    return out
+
+
+@pytest.mark.parametrize("dt", ["f32", "f64"])
+def test_gradient_tag_threshold_boundaries(sx, dt):
+    """|grad u| >= t at the rounding boundary: two-cell rows [0, g) with no
+    vertical neighbours give gx = g exactly and gx^2 + gy^2 = fl(g^2), so g
+    within a few ulps of t puts the sum on both sides of the smallest s with
+    sqrt(s) >= t; plus thresholds 0, -1, 1e-30, inf and nan.  Segment table,
+    row groups and the oracle (sqrt(s) >= t, correctly rounded) agree."""
+    import torch
+
+    from subsetix_amd.interval_stencil import build_stencil_plan
+    from subsetix_amd.levels import gradient_tag
+
+    npdt = np.float32 if dt == "f32" else np.float64
+    tdt = torch.float32 if dt == "f32" else torch.float64
+    for t in (0.02, 0.3, 1.7, 1e-30, 0.0, -1.0, np.inf, np.nan):
+        gs = _ulp_neighbours(t, npdt) if np.isfinite(t) and t > 0 else [npdt(0), npdt(1e-20), npdt(0.5), npdt(3e38)]
+        gs = [g for g in gs if np.isfinite(g)] + [npdt(np.inf)]
+        nrow = len(gs)
+        # row 2k holds [0, 2) with values (0, g_k); odd rows are empty
+        off = [0]
+        for k in range(2 * nrow):
+            off.append(off[-1] + (1 if k % 2 == 0 else 0))
+        s = ox.make(off, [0] * nrow, [2] * nrow)
+        vals = np.zeros(2 * nrow, npdt)
+        vals[1::2] = gs
+        f = _field(sx, s, vals, tdt)
+        exp = ox.grad_tag(s, vals, float(t))
+        plan = build_stencil_plan(f.interval_set, f.interval_cell_offsets, 4)
+        for p in (plan, None):
+            got = to_host(gradient_tag(f, float(t), plan=p))
+            print("threshold boundary", dt, t, "seg" if p is not None else "rows", flush=True)
+            for name in ("rows", "off", "beg", "end"):
+                np.testing.assert_array_equal(getattr(got, name), getattr(exp, name), err_msg=f"{t} {name}")
+        if np.isfinite(t) and t > 0:
+            # the boundary is crossed: some g tag, some do not
+            tagged = np.diff(exp.off)[::2]
+            assert 0 < tagged.sum() < nrow, t

@@ -89,3 +89,28 @@ def test_results_alone_equal_in_mix():
             np.testing.assert_array_equal(off, exp.off)
             np.testing.assert_array_equal(b, exp.beg)
             np.testing.assert_array_equal(e, exp.end)
+
+
+@pytest.mark.gpu
+def test_query_batch_of_several_vs_oracle():
+    """Queries stacked into one batch: each query's slice of the stacked
+    output against the oracle (the stacked CSR holds no gap rows)."""
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    from oracle import oracle as ox
+    from subsetix_amd.serve import query_request, run_queries
+
+    dev = torch.device("cuda", torch.cuda.current_device())
+    reqs = [query_request(k, 0.0, rows=r, seed=100 + 7 * k) for k, r in enumerate((300, 17, 1024, 5))]
+    outs = run_queries(reqs, dev)
+    for r, o in zip(reqs, outs):
+        A = ox.make(r.payload["a"]["off"], r.payload["a"]["begin"], r.payload["a"]["end"])
+        B = ox.make(r.payload["b"]["off"], r.payload["b"]["begin"], r.payload["b"]["end"])
+        for op in ("union", "intersection", "difference"):
+            exp = ox.setop(op, A, B)
+            off, b, e = o[op]
+            np.testing.assert_array_equal(off, exp.off)
+            np.testing.assert_array_equal(b, exp.beg)
+            np.testing.assert_array_equal(e, exp.end)
