@@ -86,6 +86,25 @@ int32_t stx_setop(int32_t op, const int32_t* a_beg, const int32_t* a_end, const 
                   int64_t scratch_bytes, uint32_t epoch, hipStream_t stream);
 int64_t stx_setop_scratch_bytes(int64_t nrows_bound, int32_t group);
 
+/* Allocation-free set operation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same result as
+ * stx_setop with no maps, no transforms, d_nrows = NULL and group 0 (the
+ * flat-tile kernel), but the look-back state lives in a static 1 MiB device
+ * array of the library (one per GPU, held until the process ends) instead of
+ * a caller scratch; the caller passes only inputs and outputs (status is
+ * OR-ed as for stx_setop).  Operands needing more than the array's 131,064
+ * look-back tiles are rejected with STX_EINVAL.  Every call on one GPU shares
+ * that GPU's array: calls that overlap on one GPU (other streams or threads)
+ * race.  Allocates nothing (no hipMalloc, hipFree or allocating library call).
+ */
+int32_t stx_setop_noalloc(int32_t op, const int32_t* a_beg, const int32_t* a_end, const int32_t* a_off, int64_t a_n,
+                          const int32_t* b_beg, const int32_t* b_end, const int32_t* b_off, int64_t b_n, int64_t nrows,
+                          int32_t* out_off, int32_t* out_beg, int32_t* out_end, int64_t capacity, int32_t* d_total,
+                          uint32_t* status, hipStream_t stream);
+/* Size in bytes of that static array (the operand limit above). */
+int64_t stx_setop_noalloc_max_scratch_bytes(void);
+
 /* Sorted union of two row-id lists (shifted) with source maps: replaces the
  * cp.union1d + searchsorted of _align_interval_sets / _reindex_interval_set
  * (expressions.py:58-141, :138).  tmp: 2*(nb+1) int32; outputs sized na+nb;

@@ -107,6 +107,11 @@ def set_group_override(g: Optional[int]) -> None:
     _GROUP_OVERRIDE = g
 
 
+_lib.register("stx_setop_noalloc", _I32, [_I32, _P, _P, _P, _I64, _P, _P, _P, _I64, _I64, _P, _P, _P, _I64, _P, _P,
+                                          _P])
+_lib.register("stx_setop_noalloc_max_scratch_bytes", _I64, [])
+
+
 def setop_raw(op: int, a, b, *, nrows: int, d_nrows: Optional[torch.Tensor] = None,
               map_a=None, map_b=None, xf_a=(XF_NONE, 0), xf_b=(XF_NONE, 0),
               capacity: Optional[int] = None, buffers=None, group: Optional[int] = None):

@@ -27,6 +27,8 @@
 
 #include <limits.h>
 
+#include <mutex>
+
 using namespace stx;
 
 namespace {
@@ -1223,6 +1225,60 @@ static int32_t launch_setop_g(int32_t op, const Operand& A, const Operand& B, co
     return launch_fused_rows<G>(SetopRow<2>{A, B}, L, "setop_difference");
 }
 
+// ---- allocation-free variant -------------------------------------------------
+// For callers that cannot allocate (code under graph capture, tight serving
+// loops): the look-back state of stx_setop_noalloc lives in this file-scope
+// device array instead of a caller workspace.  Each GPU holds its own copy
+// (zero at module load) until the process ends; its address on a device is
+// taken once, with that device current, through hipGetSymbolAddress and
+// kept per device.  Launches are told apart by a per-device epoch counter
+// (as the caller-scratch path does), so nothing is cleared between calls;
+// at epoch wrap the array is zeroed by a fill kernel.  Calls that overlap on
+// one GPU (other streams, other threads) share the array and race.
+constexpr int64_t kNoAllocScratchBytes = 1 << 20;  // 64-byte header + 131,064 tile states
+constexpr int kNoAllocMaxDevices = 64;
+constexpr uint32_t kEpochLimit = 1u << 30;
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+__global__ void zero_u64_kernel(unsigned long long* p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0ull;
+}
+
+struct NoAllocState {
+    void* ptr = nullptr;  // device address of `scratch` on this device
+    uint32_t epoch = 0;   // last epoch handed out
+};
+static NoAllocState g_noalloc[kNoAllocMaxDevices];
+static std::mutex g_noalloc_mu;
+
+// Device address of the static array on the current device and the epoch
+// for the next launch (zeroing the array on `stream` first when the epoch
+// wraps).  Returns a status code.
+static int32_t noalloc_acquire(hipStream_t stream, void** ptr, uint32_t* epoch) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kNoAllocMaxDevices)
+        return stx_fail(STX_EHIP, "setop_noalloc: no current HIP device (or device id above 63)");
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    NoAllocState& st = g_noalloc[dev];
+    if (st.ptr == nullptr) {
+        void* p = nullptr;
+        const hipError_t rc = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (rc != hipSuccess || p == nullptr)
+            return stx_fail(STX_EHIP, "setop_noalloc: hipGetSymbolAddress of the static scratch failed");
+        st.ptr = p;
+    }
+    if (st.epoch + 1 >= kEpochLimit) {
+        hipLaunchKernelGGL(zero_u64_kernel, dim3(256), dim3(256), 0, stream, (unsigned long long*)st.ptr,
+                           kNoAllocScratchBytes / 8);
+        st.epoch = 0;
+    }
+    *ptr = st.ptr;
+    *epoch = ++st.epoch;
+    return STX_OK;
+}
+
 extern "C" {
 
 int32_t stx_scan_i32(const int32_t* in, const int32_t* d_n, int64_t n_bound, int32_t* out, int32_t* d_total,
@@ -1343,6 +1399,36 @@ int32_t stx_rows_union(const int32_t* rows_a, int64_t na, int32_t shift_a, const
                            shift_a, rows_b, nb, shift_b, flag, pos, rows_u, map_a, map_b);
     hipLaunchKernelGGL(count_plus_kernel, dim3(1), dim3(1), 0, stream, d_nu, pos + nb, (int32_t)na);
     return stx_launch_status("rows_union");
+}
+
+
+int64_t stx_setop_noalloc_max_scratch_bytes(void) { return kNoAllocScratchBytes; }
+
+int32_t stx_setop_noalloc(int32_t op, const int32_t* a_beg, const int32_t* a_end, const int32_t* a_off, int64_t a_n,
+                          const int32_t* b_beg, const int32_t* b_end, const int32_t* b_off, int64_t b_n, int64_t nrows,
+                          int32_t* out_off, int32_t* out_beg, int32_t* out_end, int64_t capacity, int32_t* d_total,
+                          uint32_t* status, hipStream_t stream) {
+    if (op < 0 || op > 2 || nrows < 0 || a_off == nullptr || a_n < 0 || b_n < 0)
+        return stx_fail(STX_EINVAL, "setop_noalloc: bad arguments");
+    if (op != 0 && b_off == nullptr)
+        return stx_fail(STX_EINVAL, "setop_noalloc: intersection/difference need two operands");
+    const Operand A{a_beg, a_end, a_off, nullptr, XForm{XF_NONE, 0}};
+    const Operand B{b_beg, b_end, b_off, nullptr, XForm{XF_NONE, 0}};
+    const int64_t elems = a_n + (b_off ? b_n : 0);
+    // the flat 1024 x 8 tiles of stx_setop's default path
+    const int32_t rpt = setop_tile_rows<1024, 8>(elems, nrows, kFlatFill, flat_rows_per_tile<1024>());
+    const int64_t ntiles = (nrows + rpt) / rpt;
+    if (lookback_bytes(ntiles) > kNoAllocScratchBytes)
+        return stx_fail(STX_EINVAL, "setop_noalloc: the operands need more look-back tiles than the 1 MiB static "
+                                    "scratch holds (131,064 tiles of up to 8,192 intervals); use stx_setop with a "
+                                    "caller scratch");
+    void* sp = nullptr;
+    uint32_t epoch = 0;
+    const int32_t rc = noalloc_acquire(stream, &sp, &epoch);
+    if (rc) return rc;
+    launch_flat<1024, 8>(op, (uint32_t)ntiles, stream, A, B, nrows, rpt, out_off, out_beg, out_end, capacity, d_total,
+                         status, sp, epoch, stx_spin_limit());
+    return stx_launch_status("setop_flat_kernel(noalloc)");
 }
 
 }  // extern "C"
