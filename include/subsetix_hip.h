@@ -131,6 +131,15 @@ int32_t stx_rows_lookup(const int32_t* rows, int64_t R, int32_t y0, int64_t H, i
 int32_t stx_compact_rows(const int32_t* rows, const int32_t* off, int64_t R, int32_t shift, int32_t* rows_out,
                          int32_t* off_out, int32_t* d_count, int32_t* tmp, uint32_t* status, void* scratch,
                          int64_t scratch_bytes, uint32_t epoch, hipStream_t stream);
+/* Point location: replaces the locate_cells kernel of
+ * locate_interval_cells (interval_field.py:159-297) and the host searches of
+ * _locate_interval / get_cell / set_cell (interval_field.py:92-156).
+ * out_interval[q]: interval holding (qrow[q], qcol[q]) or -1; out_cell[q]
+ * (optional; needs the cell offsets co): value index, -1 for an inactive
+ * cell of an existing row, -2 for an absent row.  Either output may be NULL. */
+int32_t stx_locate_cells(const int32_t* rows, const int32_t* off, const int32_t* beg, const int32_t* end,
+                         const int32_t* co, int64_t R, const int32_t* qrow, const int32_t* qcol, int64_t nq,
+                         int32_t* out_interval, int32_t* out_cell, hipStream_t stream);
 /* Coarse rows floor(row/ratio) of restrict_set (multilevel.py:284-316) and
  * per-phase fine source rows maps[ph*cap + c]. */
 int32_t stx_coarse_rows(const int32_t* rows, const int32_t* off, int64_t R, int32_t ratio, int32_t all_rows,

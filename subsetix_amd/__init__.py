@@ -19,7 +19,7 @@ from .expressions import (
     make_symmetric_difference,
     make_union,
 )
-from .interval_field import IntervalField, create_interval_field, get_cell, set_cell
+from .interval_field import IntervalField, create_interval_field, get_cell, locate_interval_cells, set_cell
 from .interval_stencil import step_upwind_interval, step_upwind_interval_field
 from .morphology import (
     boundary_for_direction,
@@ -62,6 +62,7 @@ __all__ = [
     "IntervalField",
     "create_interval_field",
     "get_cell",
+    "locate_interval_cells",
     "set_cell",
     "MultiLevel2D",
     "MultiLevelField2D",

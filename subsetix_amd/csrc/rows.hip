@@ -159,6 +159,43 @@ __global__ void add_kernel(const int32_t* __restrict__ src, int32_t* __restrict_
     if (k < n) dst[k] = src[k] + v;
 }
 
+
+// Point location (replaces locate_cells, interval_field.py:159-297): one
+// lane per query; binary search of the query row among the row ids, then of
+// the column among that row's intervals.  out_interval[q] = index of the
+// interval holding (row, col), or -1 (the reference's result);
+// out_cell[q] (optional, needs co) = index of the cell's value, -1 when the
+// row exists but the cell is inactive, -2 when the row is absent.
+__global__ void locate_cells_kernel(const int32_t* __restrict__ rows, const int32_t* __restrict__ off,
+                                    const int32_t* __restrict__ beg, const int32_t* __restrict__ end,
+                                    const int32_t* __restrict__ co, int64_t R, const int32_t* __restrict__ qrow,
+                                    const int32_t* __restrict__ qcol, int64_t nq, int32_t* __restrict__ out_interval,
+                                    int32_t* __restrict__ out_cell) {
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t y = qrow[q], x = qcol[q];
+        int64_t lo = 0, hi = R;  // first row id >= y
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (rows[mid] < y) lo = mid + 1; else hi = mid;
+        }
+        int32_t found = -1, cell = -2;
+        if (lo < R && rows[lo] == y) {
+            cell = -1;
+            int32_t a = off[lo], b = off[lo + 1];  // first interval with end > x
+            while (a < b) {
+                const int32_t mid = a + ((b - a) >> 1);
+                if (end[mid] <= x) a = mid + 1; else b = mid;
+            }
+            if (a < off[lo + 1] && beg[a] <= x) {
+                found = a;
+                if (co) cell = co[a] + (x - beg[a]);
+            }
+        }
+        if (out_interval) out_interval[q] = found;
+        if (out_cell) out_cell[q] = cell;
+    }
+}
+
 }  // namespace
 
 extern "C" int32_t stx_scan_i32(const int32_t* in, const int32_t* d_n, int64_t n_bound, int32_t* out, int32_t* d_total,
@@ -248,6 +285,19 @@ int32_t stx_fill_i32(int32_t* p, int64_t n, int32_t v, int32_t step, hipStream_t
 int32_t stx_add_i32(const int32_t* src, int32_t* dst, int64_t n, int32_t v, hipStream_t stream) {
     if (n > 0) hipLaunchKernelGGL(add_kernel, dim3(nblk(n)), dim3(kT), 0, stream, src, dst, n, v);
     return stx_launch_status("add_i32");
+}
+
+
+int32_t stx_locate_cells(const int32_t* rows, const int32_t* off, const int32_t* beg, const int32_t* end,
+                         const int32_t* co, int64_t R, const int32_t* qrow, const int32_t* qcol, int64_t nq,
+                         int32_t* out_interval, int32_t* out_cell, hipStream_t stream) {
+    if (R < 0 || nq < 0 || (out_cell && !co)) return stx_fail(STX_EINVAL, "locate_cells: bad arguments");
+    if (nq > 0) {
+        const int64_t blocks = nblk(nq) < 4096 ? nblk(nq) : 4096;
+        hipLaunchKernelGGL(locate_cells_kernel, dim3((unsigned)blocks), dim3(kT), 0, stream, rows, off, beg, end, co, R,
+                           qrow, qcol, nq, out_interval, out_cell);
+    }
+    return stx_launch_status("locate_cells");
 }
 
 }  // extern "C"
