@@ -596,6 +596,41 @@ class Instance3:
         return mass(self.vals[0], 1.0 / (self.n * self.n))
 
 
+def two_level_loop(u0: np.ndarray, n: int, steps: int, regrid_every: int, *, velocity=(1.0, 0.5), cfl=0.5,
+                   threshold=0.1, padding=2, mode="moore", tag="value", accumulate="f64", dtype=np.float64):
+    """simulation.py:232-354 on oracle primitives: full-domain n x n coarse
+    and 2n x 2n fine levels, ratio 2; each step synchronises, regrids every
+    ``regrid_every`` steps (value or gradient tag, grading by ``padding``), advances both
+    levels and synchronises.  Returns (coarse values, fine values, refine)."""
+    w = h = n
+    coarse, fine = full(w, h), full(2 * w, 2 * h)
+    dxc, dxf = 1.0 / w, 0.5 / w
+    dt = cfl / (abs(velocity[0]) / dxf + abs(velocity[1]) / dxf)
+
+    def refine_of(cv):
+        d = cv.reshape(h, w).astype(np.float32)
+        tagged = gradient_tag(d, w, h, threshold) if tag == "gradient" else threshold_runs(d, w, h, threshold)
+        return intersection(grading(tagged, padding=padding, mode=mode, width=w, height=h), coarse)
+
+    cv = np.asarray(u0, dtype=dtype).reshape(-1).copy()
+    _, fv = prolong_field(coarse, cv, 2, dtype=dtype)
+    refine = refine_of(cv)
+    for step in range(steps):
+        cv, fv = synchronize(coarse, cv, fine, fv, refine, ratio=2, accumulate=accumulate)
+        if step % regrid_every == 0:
+            refine = refine_of(cv)
+        cv = upwind(coarse, cv, width=w, height=h, a=velocity[0], b=velocity[1], dt=dt, dx=dxc, dy=dxc)
+        fv = upwind(fine, fv, width=2 * w, height=2 * h, a=velocity[0], b=velocity[1], dt=dt, dx=dxf, dy=dxf)
+        cv, fv = synchronize(coarse, cv, fine, fv, refine, ratio=2, accumulate=accumulate)
+    return cv, fv, refine
+
+
+def threshold_runs(data, width, height, t):
+    """regrid.py:277-333 with the max(t, 1e-8) floor of the value tag
+    (regrid.py:386-401)."""
+    return threshold(data, width, height, max(float(t), 1e-8))
+
+
 def set_threads(n: int) -> None:
     lib().ox_set_threads(ctypes.c_int(n))
 

@@ -171,27 +171,8 @@ def _oracle_amr(u0, n, steps, regrid_every, vel, cfl, thr, grading, mode, tag, a
                 return_refine=False):
     """simulation.py:232-354 restated with oracle primitives (full-domain
     levels, ratio 2)."""
-    w = h = n
-    coarse, fine = ox.full(w, h), ox.full(2 * w, 2 * h)
-    dxc, dxf = 1.0 / w, 0.5 / w
-    dt = cfl / (abs(vel[0]) / dxf + abs(vel[1]) / dxf)
-
-    def refine_of(cv):
-        d = cv.reshape(h, w).astype(np.float32)
-        tagged = ox.gradient_tag(d, w, h, thr) if tag == "gradient" else ox.threshold(d, w, h, max(thr, 1e-8))
-        graded = ox.grading(tagged, padding=grading, mode=mode, width=w, height=h)
-        return ox.intersection(graded, coarse)
-
-    cv = u0.astype(dtype).copy()
-    _, fv = ox.prolong_field(coarse, cv, 2, dtype=dtype)
-    refine = refine_of(cv)
-    for step in range(steps):
-        cv, fv = ox.synchronize(coarse, cv, fine, fv, refine, ratio=2, accumulate=accumulate)
-        if step % regrid_every == 0:
-            refine = refine_of(cv)
-        cv = ox.upwind(coarse, cv, width=w, height=h, a=vel[0], b=vel[1], dt=dt, dx=dxc, dy=dxc)
-        fv = ox.upwind(fine, fv, width=2 * w, height=2 * h, a=vel[0], b=vel[1], dt=dt, dx=dxf, dy=dxf)
-        cv, fv = ox.synchronize(coarse, cv, fine, fv, refine, ratio=2, accumulate=accumulate)
+    cv, fv, refine = ox.two_level_loop(u0, n, steps, regrid_every, velocity=vel, cfl=cfl, threshold=thr,
+                                       padding=grading, mode=mode, tag=tag, accumulate=accumulate, dtype=dtype)
     return (cv, fv, refine) if return_refine else (cv, fv)
 
 
