@@ -337,6 +337,52 @@ int64_t stx_threshold_scratch_bytes(int32_t height);
 /* subsetix_gradient_magnitude_flat (regrid.py:175-223) as a separate pass. */
 int32_t stx_gradient(const float* data, int32_t width, int32_t height, float* out, hipStream_t stream);
 
+
+/* ---- batched three-level AMR with a device-only regrid ---------------------
+ * Replaces the host-driven regrid of simulation.py:291-328 / regrid.py:404-442
+ * (tag, grade, regrid) and the per-level step (simulation.py:335-354) for a
+ * stack of independent instances, with launch shapes fixed by the instance
+ * geometry so that a regrid and the steps after it are captured once as one
+ * graph.  Levels are dense row-major per instance; the refine sets are bit
+ * masks (one uint64 per 64-cell row chunk); work lists are chunk descriptors
+ * (i << 48 | y << 24 | m) with device-side counts (d_n; NULL = bound).
+ * lt* : int64[4 n] per level (side, first cell, first mask word, words per
+ * row).  t: tag threshold (|grad u| >= t). */
+int32_t stx_amr3_dilate(const int64_t* lt, const uint64_t* list, const int32_t* d_n, int64_t bound, const uint64_t* in,
+                        uint64_t* out, hipStream_t stream);
+int32_t stx_amr3_refine(const int64_t* ltp, const int64_t* ltc, const uint64_t* list, const int32_t* d_n, int64_t bound,
+                        const uint64_t* rp, const uint64_t* d, uint64_t* r, hipStream_t stream);
+int32_t stx_amr3_list_count(int32_t mode, const int64_t* ltp, const int64_t* ltc, const uint64_t* list, const int32_t* d_n,
+                            int64_t bound, const uint64_t* e, int32_t* counts, hipStream_t stream);
+int32_t stx_amr3_list_emit(int32_t mode, const int64_t* ltp, const int64_t* ltc, const uint64_t* list, const int32_t* d_n,
+                           int64_t bound, const uint64_t* e, const int32_t* offsets, uint64_t* out, hipStream_t stream);
+int32_t stx_amr3_zero(const int64_t* lt, const uint64_t* list, const int32_t* d_n, int64_t bound, uint64_t* m0,
+                      uint64_t* m1, uint64_t* m2, hipStream_t stream);
+int32_t stx_amr3_popc(const int64_t* lt, const uint64_t* list, const int32_t* d_n, int64_t bound, const uint64_t* m,
+                      uint64_t* out, hipStream_t stream);
+/* Work records (built once per regrid; see amr3.hip): kind 0 = stencil
+ * records of the listed chunks of level ltc holding cells of prolong(rp),
+ * kind 1 = restriction records of the listed chunks of level ltp whose refine
+ * word (rp) is non-zero.  Record-driven step and regrid kernels read one
+ * coalesced record per chunk. */
+int32_t stx_amr3_rec_count(int32_t kind, const int64_t* ltp, const int64_t* ltc, const uint64_t* list, const int32_t* d_n,
+                           int64_t bound, const uint64_t* rp, int32_t* counts, hipStream_t stream);
+int32_t stx_amr3_rec_emit(int32_t kind, const int64_t* ltp, const int64_t* ltc, const uint64_t* list, const int32_t* d_n,
+                          int64_t bound, const uint64_t* rp, const int32_t* offsets, uint64_t* out, hipStream_t stream);
+int32_t stx_amr3_tag0_records(const int64_t* lt0, const uint64_t* list, int64_t n, uint64_t* out, hipStream_t stream);
+int32_t stx_amr3_tag0_rec(int32_t dtype, const uint64_t* rec, int64_t n, const void* v, double t, uint64_t* out,
+                          hipStream_t stream);
+int32_t stx_amr3_stencil_rec(int32_t dtype, const uint64_t* rec1, const int32_t* d_n1, const uint64_t* rec2,
+                             const int32_t* d_n2, int64_t bound, const void* v0, const void* v1, const void* v2, void* v1n,
+                             void* v2n, const void* coef1, const void* coef2, hipStream_t stream);
+int32_t stx_amr3_lupdate_rec(int32_t dtype, const int64_t* ltp, const int64_t* ltc, const uint64_t* rec, const int32_t* d_n,
+                             int64_t bound, const uint64_t* rold, const void* vp, void* vc, hipStream_t stream);
+int32_t stx_amr3_tagl_rec(int32_t dtype, const int64_t* ltc, const uint64_t* rec, const int32_t* d_n, int64_t bound,
+                          const void* v, double t, uint64_t* out, hipStream_t stream);
+int32_t stx_amr3_restrict_rec(int32_t dtype, int32_t acc64, const uint64_t* rec, const int32_t* d_n, int64_t bound,
+                              const void* vc, void* vp, hipStream_t stream);
+int32_t stx_amr3_accumulate(uint64_t* total, const uint64_t* bits, int64_t base, int64_t steps, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -491,27 +491,6 @@ __device__ __forceinline__ bool grad_tag_cell(T center, T left, T right, T down,
     return gx2 + gy2 >= smin;
 }
 
-// Smallest s >= 0 whose correctly rounded square root is >= t (host side,
-// binary search over the non-negative bit patterns); NaN for a NaN t.
-template <typename T, typename Bits>
-T sqrt_threshold(T t) {
-    if (t != t) return t;
-    if (!(t > (T)0)) return (T)0;
-    Bits lo = 0, hi;
-    const T inf = std::numeric_limits<T>::infinity();
-    __builtin_memcpy(&hi, &inf, sizeof(T));
-    while (lo < hi) {
-        const Bits mid = lo + (hi - lo) / 2;
-        T v;
-        __builtin_memcpy(&v, &mid, sizeof(T));
-        if (std::sqrt(v) >= t) hi = mid;
-        else lo = mid + 1;
-    }
-    T r;
-    __builtin_memcpy(&r, &lo, sizeof(T));
-    return r;
-}
-
 // Tag and left-neighbour bits of one group [c, c + G) (bit v = cell c + v),
 // in two steps as UpwindGroup: issue() finds the segment and loads the rows
 // below and above; finish() computes from the own values cv and the cells
