@@ -14,8 +14,9 @@ Also reported beside the headline (each in its own object):
   plain_form     one pair through the reference-shaped evaluate() call
   morphology     dilate / erode, square SE radius 1, 2, 4 (one pair)
   large_stencil  8192^2 fine / 4096^2 coarse composite, 500 steps, FP32 vs FP64
-  batched_amr    512 three-level instances, 200 steps, regrid every 8, hipGraph
-                 steps, vs the plain single-instance form on a sample
+  batched_amr    512 three-level instances, 200 steps, regrid every 8, device
+                 regrid, each regrid + 8 steps one hipGraph replay; exclusive
+                 regrid / step GPU times; vs the plain single-instance form
   plain_amr      reference-shaped two-level run, 128^2 Gaussian bump, FP64
   serve          served stream (Poisson arrivals, continuous batching under an
                  HBM budget) vs the unbatched plain loop: throughput, p50/p99
@@ -195,7 +196,8 @@ def cpu_batched_amr(dev, steps=10):
 
     from oracle import oracle as ox
     from subsetix_amd import workloads
-    from subsetix_amd.batch import BatchConfig, BatchedAMR
+    from subsetix_amd.amr3 import DeviceBatchedAMR
+    from subsetix_amd.batch import BatchConfig
 
     insts = workloads.batched_instances(count=512, seed=4)
     ns = np.array([i["n"] for i in insts])
@@ -217,8 +219,9 @@ def cpu_batched_amr(dev, steps=10):
     (o32, cells), t_all = _with_threads(ox, ox.num_threads(), lambda: oracle_run(np.float32))
     (_, c1), t_one = _with_threads(ox, 1, lambda: oracle_run(np.float32))
     o64, _ = oracle_run(np.float64)
-    eng = BatchedAMR(sub, dtype=torch.float32, config=BatchConfig(levels=3, regrid_every=8, threshold=(0.02, 0.02),
-                                                                    grading=1), device=dev)
+    eng = DeviceBatchedAMR(sub, dtype=torch.float32, config=BatchConfig(levels=3, regrid_every=8,
+                                                                          threshold=(0.02, 0.02), grading=1),
+                           device=dev)
     eng.run(steps)
     cs = eng.cstart0.cpu().numpy()
     lv0 = eng.level0_values().cpu().numpy()

@@ -47,7 +47,7 @@ from typing import Dict, List, Optional
 import numpy as np
 import torch
 
-from . import _core, _lib
+from . import _core, _lib, autotune
 from ._lib import _F64, _I32, _I64, _P, check, context, ptr
 from .batch import BatchConfig
 from .expressions import IntervalSet
@@ -182,6 +182,11 @@ class DeviceBatchedAMR:
         self._gstream = torch.cuda.Stream(dev)
         self.regrids = 0
         self.regrid()
+        if self.cfg.autotune:
+            # one step's kernels under the schedule autotuner; the state does
+            # not move (the step reads buffer cur and writes the other one)
+            with autotune.tuning():
+                self._step(self.cur)
 
     # ------------------------------------------------------------------ setup
     def _level0_plan(self) -> None:

@@ -5,7 +5,7 @@ device with events; every runner returns a plain dict).
   * setalgebra_plain    - one pair through the reference-shaped API
   * morphology          - dilate / erode (square SE r = 1, 2, 4), one pair
   * large_stencil       - 8192^2 fine / 4096^2 coarse composite, FP32 vs FP64
-  * batched_amr         - many 3-level instances, hipGraph steps
+  * batched_amr         - many 3-level instances, device regrid, hipGraph cycles
   * plain_two_level     - the reference-shaped two-level driver (128^2 bump)
 """
 
@@ -13,6 +13,7 @@ from __future__ import annotations
 
 import sys
 import time
+import warnings
 from typing import Dict, List
 
 import numpy as np
@@ -240,54 +241,77 @@ def large_stencil(steps: int, dev, n: int = 8192, seed: int = 3) -> Dict:
 
 def batched_amr(count: int, steps: int, dev, *, plain_sample: int = 16, plain_steps: int = 16,
                 seed: int = 4) -> Dict:
-    """Many 3-level instances: batched (one launch per kernel per step,
-    hipGraph replay) vs the plain form (one instance at a time, eager
-    launches, a host synchronisation per step), both counting the level
-    cells advanced per step (ghost cells excluded).  The FP32 masses of a
-    sample of instances are compared with an FP64 batched run of the same
-    instances (batch == alone holds bit-exactly, so the sample stands for
-    its instances inside the full batch)."""
+    """Many 3-level instances through the device-regrid engine
+    (subsetix_amd.amr3: each regrid + 8 steps is one hipGraph replay, no
+    host synchronisation) vs the plain form (one instance at a time through
+    the interval-set engine, eager launches, a host synchronisation per
+    step), both counting the level-set cells advanced per step (ghost cells
+    excluded).  Also: the regrid's exclusive GPU time and a step's, timed
+    with events around eager launches; the timed run under torch's sync
+    debug mode set to 'error' (any host read raises); FP32 masses of a sample
+    against an FP64 run of the same instances (instances are independent,
+    so a sample stands for itself inside the full batch)."""
+    from .amr3 import DeviceBatchedAMR
     from .batch import BatchConfig, BatchedAMR
 
     insts = workloads.batched_instances(count=count, seed=seed)
     cfg = BatchConfig(levels=3, regrid_every=8, threshold=(0.02, 0.02), grading=1, autotune=True)
     k = cfg.regrid_every
     t0 = time.perf_counter()
-    eng = BatchedAMR(insts, config=cfg, device=dev)
+    eng = DeviceBatchedAMR(insts, config=cfg, device=dev)
+    torch.cuda.synchronize()
     setup_s = time.perf_counter() - t0
-    log(f"batched_amr: {count} instances set up in {setup_s:.1f}s, cells {eng.cells}")
+    log(f"batched_amr: {count} instances set up in {setup_s:.1f}s, cells {eng.cells()}")
     m0 = eng.masses().cpu().numpy()
-
-    def advance(e, n_steps):
-        cells, done = 0, 0
-        while done < n_steps:
-            n = min(k, n_steps - done)
-            e.run(n)  # regrids first when the step count is a multiple of k
-            c = e.cells
-            cells += (c["level0"] + c["level1"] + c.get("level2", 0)) * n
-            done += n
-        return cells
-
-    eng.run(2)  # warm-up: graph capture + first replays
-    eng2_steps = steps
-    regrid0 = eng.timings.get("regrid_s", 0.0)
+    eng.run(2 * k)  # warm-up: both graphs captured (one per refine-mask parity) and replayed
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    cells_total = advance(eng, eng2_steps)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    regrid_s = eng.timings.get("regrid_s", 0.0) - regrid0
-    log(f"batched_amr: {eng2_steps} steps in {wall:.2f}s (regrid {regrid_s:.2f}s)")
+    c0 = eng.cell_updates()
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")  # the mode is labelled a prototype feature
+        torch.cuda.set_sync_debug_mode("error")
+    try:
+        t0 = time.perf_counter()
+        eng.run(steps)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+    finally:
+        torch.cuda.set_sync_debug_mode("default")
+    cells_total = eng.cell_updates() - c0
+    log(f"batched_amr: {steps} steps in {wall:.3f}s")
     m1 = eng.masses().cpu().numpy()
-    total_steps = 2 + eng2_steps
+    total_steps = 2 * k + steps
+    cells_last = eng.cells()
+    # exclusive GPU times: a regrid (eager, on the settled state: it rebuilds
+    # the same sets) and a step (eager launches; the state moves one step)
+    s_ev, e_ev = _events()
+    for _ in range(2):
+        eng.regrid()
+    s_ev.record()
+    for _ in range(10):
+        eng.regrid()
+    e_ev.record()
+    torch.cuda.synchronize()
+    regrid_ms = s_ev.elapsed_time(e_ev) / 10
+    s_ev.record()
+    eng._steps(k)
+    e_ev.record()
+    torch.cuda.synchronize()
+    step_ms = s_ev.elapsed_time(e_ev) / k
     idx = list(range(0, count, max(1, count // plain_sample)))[:plain_sample]
     m1_sample = m1[idx]
     res = {
-        "instances": count, "steps": eng2_steps, "setup_s": setup_s, "wall_s": wall, "regrid_s": regrid_s,
-        "ms_per_step": wall * 1e3 / eng2_steps,
-        "cells_last": eng.cells, "cell_updates": cells_total,
+        "engine": "device regrid: dense levels, bit-mask refine sets, per-chunk work records, one graph per "
+                  "regrid + 8 steps",
+        "instances": count, "steps": steps, "setup_s": setup_s, "wall_s": wall,
+        "ms_per_step": wall * 1e3 / steps,
+        "cells_last": cells_last, "cell_updates": cells_total,
         "cell_updates_per_s": cells_total / wall,
         "coarse_cells": int((eng.sizes.astype(np.int64) ** 2).sum()),
+        "regrid_gpu_ms": regrid_ms, "step_gpu_ms": step_ms,
+        "regrid_share": regrid_ms / (regrid_ms + k * step_ms),
+        "host_reads_per_regrid": 0,
+        "host_reads_check": "timed run under torch.cuda.set_sync_debug_mode('error'); regrid and steps issue "
+                            "only asynchronous launches (no device-to-host copy in either)",
     }
     drift = np.abs((m1 - m0) / np.where(m0 != 0, m0, 1.0))
     res["mass_change_rel_max"] = float(drift.max())
@@ -295,7 +319,9 @@ def batched_amr(count: int, steps: int, dev, *, plain_sample: int = 16, plain_st
     torch.cuda.empty_cache()
     # FP64 reference of the sampled instances (same steps, same regrid cadence)
     sub = [insts[i] for i in idx]
-    e64 = BatchedAMR(sub, dtype=torch.float64, config=cfg, device=dev)
+    e64 = DeviceBatchedAMR(sub, dtype=torch.float64, config=BatchConfig(levels=3, regrid_every=8,
+                                                                         threshold=(0.02, 0.02), grading=1),
+                           device=dev)
     e64.run(total_steps)
     m64 = e64.masses().cpu().numpy()
     del e64
@@ -321,11 +347,10 @@ def batched_amr(count: int, steps: int, dev, *, plain_sample: int = 16, plain_st
         del e1
     res["plain"] = {"instances_sampled": len(idx), "steps_each": plain_steps, "cell_updates": pcells,
                     "cell_updates_per_s": pcells / ptime,
-                    "desc": "one instance at a time, eager launches, host synchronisation every step "
-                            "(regrids included, every 8 steps)"}
+                    "desc": "one instance at a time through the interval-set engine, eager launches, host "
+                            "synchronisation every step (regrids included, every 8 steps)"}
     res["batched_vs_plain"] = res["cell_updates_per_s"] / res["plain"]["cell_updates_per_s"]
-    res["autotune"] = {"step": autotune_report(("upwind", "transfer")),
-                       "choices": _choice_counts(("setop", "grad_tag", "upwind", "transfer"))}
+    res["autotune"] = {"step": autotune_report(("upwind",)), "choices": _choice_counts(("upwind",))}
     return res
 
 
