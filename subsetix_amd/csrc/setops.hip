@@ -309,6 +309,7 @@ __device__ __forceinline__ void stage_pairs(const int32_t* __restrict__ b, const
 // never wait on each other; every scratch word stays epoch-tagged.
 constexpr int kFused = 0, kCount = 1, kWrite = 2;
 
+
 template <int OP, int MODE, bool XF, int TB, int VT>
 __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(tile_waves_per_eu<VT>()))) void setop_tile_kernel(Operand A, Operand B, int64_t U, int32_t RPT, uint32_t ntiles,
                                                             int32_t* __restrict__ out_off, int32_t* __restrict__ ob,
@@ -696,11 +697,40 @@ __device__ __forceinline__ void flat_load(const Operand& A, const Operand& B, in
     }
 }
 
+// Staged-key slot i lives at i + i / 16 in LDS: one spare slot per 16
+// spreads the strided accesses of the staging and of the merge walks (lanes
+// 4 to 8 slots apart) over all banks.
+__device__ __forceinline__ int32_t sk(int32_t i) { return i + (i >> 4); }
+constexpr int sk_size(int n) { return n + n / 16 + 1; }
+
+// LDS of a flat tile's produce phase; the staged keys (s_k) live apart so
+// that a block can keep one tile's outputs while it stages the next.
+template <int TB>
+struct FlatShared {
+    // element starts of the tile's A rows [0, nr], then nA + those of its B
+    // rows at [nr + 1, 2 nr + 1], then two INT32_MAX sentinels
+    int32_t cS[2 * flat_rows_per_tile<TB>() + 4];
+    int32_t red[2 * (TB / STX_WAVE)];
+    int32_t wmax[TB / STX_WAVE];
+    int32_t wsum[TB / STX_WAVE];
+};
+
+// A produced tile: its rows, output count and key transform (uniform).
+struct FlatTile {
+    int64_t row0;
+    int32_t nr, total, xmin;
+    uint32_t shift;
+    bool flat;
+};
+
+// Stages tile `tile` and walks it, leaving its outputs compacted in s_k (a
+// tile processed one wavefront per row leaves its rows' exclusive output
+// offsets there instead), and publishes the tile's output count as its
+// look-back aggregate.  Called by every thread of the block.
 template <int OP, bool XF, int TB, int VT>
-__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(tile_waves_per_eu<VT>()))) void setop_flat_kernel(
-    Operand A, Operand B, int64_t U, int32_t RPT, uint32_t ntiles, int32_t* __restrict__ out_off,
-    int32_t* __restrict__ ob, int32_t* __restrict__ oe, int64_t capacity, int32_t* __restrict__ d_total,
-    uint32_t* status, void* scratch, uint32_t epoch, int64_t spin_limit) {
+__device__ __forceinline__ FlatTile flat_produce(const Operand& A, const Operand& B, int64_t U, int32_t RPT, uint32_t tile,
+                                                 int2* __restrict__ s_k, FlatShared<TB>& sh, LookbackScratch lb,
+                                                 uint32_t epoch) {
     constexpr int kTW = TB / STX_WAVE;
     constexpr int kPCAP = TB * VT;
     constexpr int kMaxRPT = flat_rows_per_tile<TB>();
@@ -709,17 +739,6 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(tile_waves_p
     // upper sentinels: x above every key; y below every key (the union's
     // max-scan ignores them), or above (difference: B merges by end)
     constexpr int32_t kHiY = OP == 2 ? INT32_MAX : INT32_MIN;
-    __shared__ __align__(16) int2 s_k[kPCAP];
-    // element starts of the tile's A rows [0, nr], then nA + those of its B
-    // rows at [nr + 1, 2 nr + 1], then two INT32_MAX sentinels
-    __shared__ int32_t s_cS[2 * kMaxRPT + 4];
-    __shared__ int32_t s_red[2 * kTW];
-    __shared__ int32_t s_wmax[kTW];
-    __shared__ int32_t s_wsum[kTW];
-    __shared__ uint32_t s_excl;
-
-    LookbackScratch lb = lookback_scratch(scratch, ntiles);
-    const uint32_t tile = blockIdx.x;
     const int tid = threadIdx.x, lane = lane_id(), wave = wave_id();
     const int64_t row0 = (int64_t)tile * RPT;
     const int nr = (int)(U - row0 < RPT ? (U - row0 > 0 ? U - row0 : 0) : RPT);
@@ -735,10 +754,10 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(tile_waves_p
     int32_t xmin = 0;
     if (flat) {
         for (int r = tid; r <= nr; r += TB) {
-            s_cS[r] = A.off[row0 + r] - a_base;
-            s_cS[nr + 1 + r] = nA + (B.off ? B.off[row0 + r] - b_base : 0);
+            sh.cS[r] = A.off[row0 + r] - a_base;
+            sh.cS[nr + 1 + r] = nA + (B.off ? B.off[row0 + r] - b_base : 0);
         }
-        if (tid < 2) s_cS[2 * nr + 2 + tid] = INT32_MAX;
+        if (tid < 2) sh.cS[2 * nr + 2 + tid] = INT32_MAX;
         // VT consecutive elements of the sequence A then B per thread
         const int32_t q0 = tid * VT;
         int32_t xb[VT], xe[VT];
@@ -754,14 +773,14 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(tile_waves_p
         lo = wave_last(wave_incl_min(lo));
         hi = wave_last(wave_incl_max(hi));
         if (lane == 0) {
-            s_red[wave] = lo;
-            s_red[kTW + wave] = hi;
+            sh.red[wave] = lo;
+            sh.red[kTW + wave] = hi;
         }
         __syncthreads();  // row starts, block min / max
 #pragma unroll
         for (int w = 0; w < kTW; ++w) {
-            lo = imin(lo, s_red[w]);
-            hi = imax(hi, s_red[kTW + w]);
+            lo = imin(lo, sh.red[w]);
+            hi = imax(hi, sh.red[kTW + w]);
         }
         const int64_t range = P > 0 ? (int64_t)hi - (int64_t)lo : 0;
         shift = range > 0 ? 64u - (uint32_t)__clzll((unsigned long long)range) : 0u;
@@ -769,14 +788,14 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(tile_waves_p
         xmin = lo;
         if (flat) {
             if (q0 < P) {
-                // combined row index k of element q0: s_cS[k] <= q0 < s_cS[k + 1]
+                // combined row index k of element q0: sh.cS[k] <= q0 < sh.cS[k + 1]
                 int k = 0, kh = 2 * nr + 2;
                 while (kh - k > 1) {
                     const int mid = (k + kh) >> 1;
-                    if (s_cS[mid] <= q0) k = mid;
+                    if (sh.cS[mid] <= q0) k = mid;
                     else kh = mid;
                 }
-                int32_t nxt = s_cS[k + 1];
+                int32_t nxt = sh.cS[k + 1];
                 uint32_t off = ((uint32_t)(k < nr ? k : k - nr - 1) << shift) - (uint32_t)xmin;
 #pragma unroll
                 for (int u = 0; u < VT; ++u) {
@@ -785,22 +804,22 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(tile_waves_p
                         if (q >= nxt) {
                             do {
                                 ++k;
-                                nxt = s_cS[k + 1];
+                                nxt = sh.cS[k + 1];
                             } while (q >= nxt);
                             off = ((uint32_t)(k < nr ? k : k - nr - 1) << shift) - (uint32_t)xmin;
                         }
                         const int32_t slot = q < nA ? a0 + q : b0 + (q - nA);
-                        s_k[slot] = make_int2((int32_t)((uint32_t)xb[u] + off), (int32_t)((uint32_t)xe[u] + off));
+                        s_k[sk(slot)] = make_int2((int32_t)((uint32_t)xb[u] + off), (int32_t)((uint32_t)xe[u] + off));
                     }
                 }
             }
             if (tid <= VT) {
-                s_k[a0 + nA + tid] = make_int2(INT32_MAX, kHiY);
-                s_k[b0 + nB + tid] = make_int2(INT32_MAX, kHiY);
+                s_k[sk(a0 + nA + tid)] = make_int2(INT32_MAX, kHiY);
+                s_k[sk(b0 + nB + tid)] = make_int2(INT32_MAX, kHiY);
             }
             if (tid == 0) {
-                s_k[a0 - 1] = make_int2(INT32_MIN, INT32_MIN);
-                s_k[b0 - 1] = make_int2(INT32_MIN, INT32_MIN);
+                s_k[sk(a0 - 1)] = make_int2(INT32_MIN, INT32_MIN);
+                s_k[sk(b0 - 1)] = make_int2(INT32_MIN, INT32_MIN);
             }
         }
     }
@@ -818,18 +837,18 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(tile_waves_p
             int32_t lo = p0 - nB > 0 ? p0 - nB : 0, hi = p0 < nA ? p0 : nA;
             while (lo < hi) {
                 const int32_t mid = (lo + hi) >> 1;
-                const int2 ka = s_k[a0 + mid];
-                const int2 kb = s_k[b0 + (p0 - 1 - mid)];
+                const int2 ka = s_k[sk(a0 + mid)];
+                const int2 kb = s_k[sk(b0 + (p0 - 1 - mid))];
                 if (key_a_first<OP>(ka.x, kb.x, kb.y)) lo = mid + 1;
                 else hi = mid;
             }
             int32_t ia = a0 + lo, jb = b0 + (p0 - lo);  // slots of the current A and B elements
             int32_t cAx, cAy, cBx, cBy, pAx = 0, pAy = 0, pBy = 0;
             {
-                const int2 t0 = s_k[ia], t1 = s_k[jb];
+                const int2 t0 = s_k[sk(ia)], t1 = s_k[sk(jb)];
                 cAx = t0.x, cAy = t0.y, cBx = t1.x, cBy = t1.y;
                 if (OP != 0) {
-                    const int2 t2 = s_k[ia - 1], t3 = s_k[jb - 1];
+                    const int2 t2 = s_k[sk(ia - 1)], t3 = s_k[sk(jb - 1)];
                     pAx = t2.x, pAy = t2.y, pBy = t3.y;
                 }
             }
@@ -837,7 +856,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(tile_waves_p
             for (int v = 0; v < VT; ++v) {
                 const bool isA = key_a_first<OP>(cAx, cBx, cBy);
                 const int32_t nidx = (isA ? ia : jb) + 1;
-                const int2 nraw = s_k[nidx];
+                const int2 nraw = s_k[sk(nidx)];
                 if (OP == 0) {
                     rb[v] = isA ? cAx : cBx;
                     re[v] = isA ? cAy : cBy;
@@ -886,7 +905,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(tile_waves_p
         int32_t carry = INT32_MIN;
         if (OP == 0) {
             // positions past the tile's end hold sentinels: re = INT32_MIN
-            carry = block_excl_max<kTW>(agg, s_wmax);
+            carry = block_excl_max<kTW>(agg, sh.wmax);
             int32_t pm = carry;
             if (p0 < P) {
 #pragma unroll
@@ -897,7 +916,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(tile_waves_p
             }
         }
         emit_mask &= valid;
-        const int32_t my_base = block_excl_sum<kTW>(__popc(emit_mask), s_wsum, &tile_total);
+        const int32_t my_base = block_excl_sum<kTW>(__popc(emit_mask), sh.wsum, &tile_total);
         // every walk is done: the staged inputs are free
         if (tid == 0) lookback_publish(lb, tile, (uint32_t)tile_total, epoch);
         int32_t w = my_base;
@@ -908,20 +927,20 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(tile_waves_p
 #pragma unroll
                 for (int v = 0; v < VT; ++v) {
                     if (emit_mask & (1u << v)) {
-                        s_k[w].x = rb[v];
-                        if (w > 0) s_k[w - 1].y = pm;
+                        s_k[sk(w)].x = rb[v];
+                        if (w > 0) s_k[sk(w - 1)].y = pm;
                         ++w;
                     }
                     pm = imax(pm, re[v]);
                 }
                 // the owner of the tile's last position closes the last run
-                if (P - p0 <= VT && tile_total > 0) s_k[tile_total - 1].y = pm;
+                if (P - p0 <= VT && tile_total > 0) s_k[sk(tile_total - 1)].y = pm;
             }
         } else {
 #pragma unroll
             for (int v = 0; v < VT; ++v) {
                 if (emit_mask & (1u << v)) {
-                    s_k[w] = make_int2(rb[v], re[v]);
+                    s_k[sk(w)] = make_int2(rb[v], re[v]);
                     ++w;
                 }
             }
@@ -938,21 +957,33 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(tile_waves_p
         }
         __syncthreads();
         const int32_t c = tid < nr ? s_cnt[tid] : 0;  // nr <= kMaxRPT <= TB
-        const int32_t ex = block_excl_sum<kTW>(c, s_wsum, &tile_total);
+        const int32_t ex = block_excl_sum<kTW>(c, sh.wsum, &tile_total);
         if (tid < nr) s_cnt[tid] = ex;
         if (tid == 0) lookback_publish(lb, tile, (uint32_t)tile_total, epoch);
     }
-    if (wave == 0) {
-        const uint32_t e = lookback_exclusive<1, true>(lb, tile, (uint32_t)tile_total, epoch, status, spin_limit);
-        if (lane == 0) s_excl = e;
-    }
-    __syncthreads();
-    const int64_t excl = (int64_t)s_excl;
+    return FlatTile{row0, nr, tile_total, xmin, shift, flat};
+}
+
+// Writes a produced tile at output offset `excl`: its intervals, its rows'
+// offsets, and for the last tile the total.  Called by every thread.
+template <int OP, int TB>
+__device__ __forceinline__ void flat_store(const FlatTile& ft, const Operand& A, const Operand& B, int64_t U, int64_t excl,
+                                           const int2* __restrict__ s_k, int32_t* __restrict__ out_off,
+                                           int32_t* __restrict__ ob, int32_t* __restrict__ oe, int64_t capacity,
+                                           int32_t* __restrict__ d_total, uint32_t* status) {
+    constexpr int kTW = TB / STX_WAVE;
+    const int tid = threadIdx.x, wave = wave_id();
+    const int64_t row0 = ft.row0;
+    const int nr = ft.nr;
+    const int32_t tile_total = ft.total;
+    const uint32_t shift = ft.shift;
+    const int32_t xmin = ft.xmin;
+    const bool flat = ft.flat;
     const bool over = excl + tile_total > capacity;
     if (over && tid == 0 && tile_total > 0) atomicOr(status, STX_FLAG_CAPACITY);
     if (flat) {
         for (int32_t k = tid; k < tile_total; k += TB) {
-            const int2 o = s_k[k];
+            const int2 o = s_k[sk(k)];
             const int32_t r = o.x >> shift;
             if (!over) {
                 const uint32_t boff = ((uint32_t)r << shift) - (uint32_t)xmin;
@@ -961,12 +992,12 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(tile_waves_p
             }
             // rows (row of output k - 1, row of output k] start at output k
             // (the bound keeps every store inside the tile's rows)
-            const int32_t rp = k > 0 ? (s_k[k - 1].x >> shift) : -1;
+            const int32_t rp = k > 0 ? (s_k[sk(k - 1)].x >> shift) : -1;
             const int32_t rt = r < nr ? r : nr - 1;
             for (int32_t rr = rp + 1; rr <= rt; ++rr) out_off[row0 + rr] = (int32_t)(excl + k);
         }
         // rows after the last output's row
-        const int32_t rl = tile_total > 0 ? (s_k[tile_total - 1].x >> shift) : -1;
+        const int32_t rl = tile_total > 0 ? (s_k[sk(tile_total - 1)].x >> shift) : -1;
         for (int32_t rr = rl + 1 + tid; rr < nr; rr += TB) out_off[row0 + rr] = (int32_t)(excl + tile_total);
     } else {
         const int32_t* s_cnt = (const int32_t*)s_k;
@@ -984,6 +1015,26 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(tile_waves_p
         out_off[U] = (int32_t)(excl + tile_total);
         if (d_total) *d_total = (int32_t)(excl + tile_total);
     }
+}
+
+
+template <int OP, bool XF, int TB, int VT>
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(tile_waves_per_eu<VT>()))) void setop_flat_kernel(
+    Operand A, Operand B, int64_t U, int32_t RPT, uint32_t ntiles, int32_t* __restrict__ out_off,
+    int32_t* __restrict__ ob, int32_t* __restrict__ oe, int64_t capacity, int32_t* __restrict__ d_total,
+    uint32_t* status, void* scratch, uint32_t epoch, int64_t spin_limit) {
+    __shared__ __align__(16) int2 s_k[sk_size(TB * VT)];
+    __shared__ FlatShared<TB> sh;
+    __shared__ uint32_t s_excl;
+    LookbackScratch lb = lookback_scratch(scratch, ntiles);
+    const uint32_t tile = blockIdx.x;
+    const FlatTile ft = flat_produce<OP, XF, TB, VT>(A, B, U, RPT, tile, s_k, sh, lb, epoch);
+    if (wave_id() == 0) {
+        const uint32_t e = lookback_exclusive<1, true>(lb, tile, (uint32_t)ft.total, epoch, status, spin_limit);
+        if (lane_id() == 0) s_excl = e;
+    }
+    __syncthreads();
+    flat_store<OP, TB>(ft, A, B, U, (int64_t)s_excl, s_k, out_off, ob, oe, capacity, d_total, status);
 }
 
 // One block: aggregates (flag 1) of tiles [0, ntiles) -> inclusive prefixes

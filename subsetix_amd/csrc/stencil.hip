@@ -226,6 +226,34 @@ __device__ __forceinline__ T upwind_cell(T center, T left, T right, T down, T up
     return center - dt * (t1 + t2);
 }
 
+// 1 / d when d is a positive power of two whose reciprocal is a normal
+// number (then x / d == x * (1 / d) exactly: both are the correctly rounded
+// value of the same exact scaling), else 0.  From the bits: 2^e -> 2^-e.
+__device__ __forceinline__ float pow2_recip(float d) {
+    const uint32_t u = __float_as_uint(d);
+    const uint32_t e = u >> 23;  // sign bit 0 and mantissa 0 checked below
+    return ((u & 0x007fffffu) == 0u && e >= 1u && e <= 253u) ? __uint_as_float((254u - e) << 23) : 0.0f;
+}
+__device__ __forceinline__ double pow2_recip(double d) {
+    const uint64_t u = __double_as_longlong(d);
+    const uint64_t e = u >> 52;
+    return ((u & 0x000fffffffffffffull) == 0ull && e >= 1ull && e <= 2045ull) ? __longlong_as_double((int64_t)((2046ull - e) << 52))
+                                                                              : 0.0;
+}
+
+// upwind_cell with the divisions by dx / dy as products with their exact
+// reciprocals rx / ry (powers of two only, see pow2_recip).
+template <typename T>
+__device__ __forceinline__ T upwind_cell_p2(T center, T left, T right, T down, T up, T a, T b, T dt, T rx, T ry) {
+    const T nx = (a >= (T)0) ? center - left : right - center;
+    const T ny = (b >= (T)0) ? center - down : up - center;
+    const T dudx = nx * rx;
+    const T dudy = ny * ry;
+    const T t1 = a * dudx;
+    const T t2 = b * dudy;
+    return center - dt * (t1 + t2);
+}
+
 // Gathers go through a buffer resource over the whole field: a 32-bit byte
 // offset per load, and an index of -1 (or >= C) reads 0 (out-of-range buffer
 // loads return zero), which is exactly the zero exterior of the stencil.
@@ -365,13 +393,25 @@ struct UpwindGroup {
         }
         const bool inner = bits & 8u;
         VT ov;
+        const T rx = pow2_recip(k[3]), ry = pow2_recip(k[4]);
+        if (rx != (T)0 && ry != (T)0) {
 #pragma unroll
-        for (int v = 0; v < G; ++v) {
-            const bool lok = v == 0 ? (bool)(bits & 2u) : inner;
-            const bool rok = v == G - 1 ? (bool)(bits & 4u) : inner;
-            const T left = lok ? (v == 0 ? lv : cv[v - 1]) : (T)0;
-            const T right = rok ? (v == G - 1 ? rv : cv[v + 1]) : (T)0;
-            ov[v] = upwind_cell<T>(cv[v], left, right, dv[v], uv[v], k[0], k[1], k[2], k[3], k[4]);
+            for (int v = 0; v < G; ++v) {
+                const bool lok = v == 0 ? (bool)(bits & 2u) : inner;
+                const bool rok = v == G - 1 ? (bool)(bits & 4u) : inner;
+                const T left = lok ? (v == 0 ? lv : cv[v - 1]) : (T)0;
+                const T right = rok ? (v == G - 1 ? rv : cv[v + 1]) : (T)0;
+                ov[v] = upwind_cell_p2<T>(cv[v], left, right, dv[v], uv[v], k[0], k[1], k[2], rx, ry);
+            }
+        } else {
+#pragma unroll
+            for (int v = 0; v < G; ++v) {
+                const bool lok = v == 0 ? (bool)(bits & 2u) : inner;
+                const bool rok = v == G - 1 ? (bool)(bits & 4u) : inner;
+                const T left = lok ? (v == 0 ? lv : cv[v - 1]) : (T)0;
+                const T right = rok ? (v == G - 1 ? rv : cv[v + 1]) : (T)0;
+                ov[v] = upwind_cell<T>(cv[v], left, right, dv[v], uv[v], k[0], k[1], k[2], k[3], k[4]);
+            }
         }
         if (!BATCH || k[3] != (T)0) *(VT*)(out + c) = ov;  // dx = 0: retired instance
     }

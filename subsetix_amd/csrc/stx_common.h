@@ -154,20 +154,23 @@ __device__ __forceinline__ unsigned long long state_load(const unsigned long lon
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Called by all 64 lanes of ONE wave of the block.  Publishes `agg` for tile
-// `tile` and returns the exclusive prefix of all earlier tiles.  Each poll
-// covers a window of 64 * K predecessors (lane l reads tiles
-// pos - l*K - k, k < K, nearest first), so a published prefix is found up to
-// 64 * K tiles back in one round trip.  After `spin_limit` polls the block
-// records STX_FLAG_SPIN in `status` and proceeds (its result is then invalid,
-// but every wave still exits).
 // Publishes tile `tile`'s aggregate ahead of its look-back (one thread):
 // tile 0's aggregate is already its inclusive prefix.
 __device__ __forceinline__ void lookback_publish(LookbackScratch s, uint32_t tile, uint32_t agg, uint32_t epoch) {
     state_store(&s.state[tile], epoch, tile == 0 ? 2u : 1u, agg);
 }
 
-// PUBLISHED: the aggregate was stored by lookback_publish already.
+// Called by all 64 lanes of ONE wave of the block.  Publishes `agg` for tile
+// `tile` (unless PUBLISHED: lookback_publish stored it already) and returns
+// the exclusive prefix of all earlier tiles.  One poll reads K blocks of 64
+// consecutive predecessors (block k, lane l: tile pos - 64 k - l; each block
+// one coalesced load), all issued before any is used, and folds the blocks
+// nearest first: aggregates are summed until a block holds an inclusive
+// prefix.  A block with a predecessor not yet published (before the first
+// prefix) stops the fold; the blocks folded so far stay folded and the next
+// poll starts behind them.  After `spin_limit` polls that found no progress
+// the block records STX_FLAG_SPIN in `status` and proceeds (its result is
+// then invalid, but every wave still exits).
 template <int K = 1, bool PUBLISHED = false>
 __device__ __forceinline__ uint32_t lookback_exclusive(LookbackScratch s, uint32_t tile, uint32_t agg, uint32_t epoch,
                                                        uint32_t* status, int64_t spin_limit) {
@@ -181,52 +184,43 @@ __device__ __forceinline__ uint32_t lookback_exclusive(LookbackScratch s, uint32
     int64_t pos = (int64_t)tile - 1;  // nearest predecessor not yet folded in
     int64_t spins = 0;
     while (pos >= 0) {
-        // this lane's K predecessors, nearest first
-        uint32_t flag[K], val[K];
+        unsigned long long w[K];
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            const int64_t p = pos - (int64_t)lane * K - k;
-            flag[k] = 2u;  // before tile 0: an empty prefix
-            val[k] = 0u;
-            if (p >= 0) {
-                const unsigned long long w = state_load(&s.state[p]);
-                const uint32_t hi = (uint32_t)(w >> 32);
-                flag[k] = (hi >> 2) == epoch ? (hi & 3u) : 0u;
-                val[k] = (uint32_t)w;
-            }
+            const int64_t p = pos - 64 * k - lane;
+            // before tile 0: an empty inclusive prefix
+            w[k] = p >= 0 ? state_load(&s.state[p]) : ((unsigned long long)((epoch << 2) | 2u) << 32);
         }
-        // within the lane: first prefix, readiness before it, partial sum
-        int kp = K;
-        bool blocked = false;
-        uint32_t part = 0;
-#pragma unroll
-        for (int k = K - 1; k >= 0; --k)
-            if (flag[k] == 2u) kp = k;
+        bool done = false;
+        int folded = 0;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            if (k <= kp && k < K) {
-                if (flag[k] == 0u) blocked = true;
-                part += val[k];
+            const uint32_t hi = (uint32_t)(w[k] >> 32);
+            const uint32_t flag = (hi >> 2) == epoch ? (hi & 3u) : 0u;
+            const uint64_t has_prefix = __ballot(flag == 2u);
+            const uint64_t not_ready = __ballot(flag == 0u);
+            const int first_prefix = has_prefix ? __ffsll((long long)has_prefix) - 1 : 64;
+            const uint64_t upto = first_prefix >= 63 ? ~0ull : ((1ull << (first_prefix + 1)) - 1ull);
+            if (not_ready & upto) break;
+            uint32_t contrib = lane <= first_prefix ? (uint32_t)w[k] : 0u;
+#pragma unroll
+            for (int d = 32; d >= 1; d >>= 1) contrib += __shfl_xor(contrib, d, 64);
+            excl += contrib;
+            folded = k + 1;
+            if (first_prefix < 64) {
+                done = true;
+                break;
             }
         }
-        const uint64_t has_prefix = __ballot(kp < K);
-        const uint64_t not_ready = __ballot(blocked);
-        const int first_prefix = has_prefix ? __ffsll((long long)has_prefix) - 1 : 64;
-        const uint64_t upto = first_prefix >= 63 ? ~0ull : ((1ull << (first_prefix + 1)) - 1ull);
-        if (not_ready & upto) {
+        if (done) break;
+        pos -= 64 * folded;
+        if (folded == 0) {
             if (++spins > spin_limit) {
                 if (lane == 0) atomicOr(status, STX_FLAG_SPIN);
                 break;
             }
             __builtin_amdgcn_s_sleep(1);
-            continue;
         }
-        uint32_t contrib = lane <= first_prefix ? part : 0u;
-#pragma unroll
-        for (int d = 32; d >= 1; d >>= 1) contrib += __shfl_xor(contrib, d, 64);
-        excl += contrib;
-        if (first_prefix < 64) break;
-        pos -= 64 * K;
     }
     if (lane == 0) state_store(&s.state[tile], epoch, 2u, excl + agg);
     return excl;

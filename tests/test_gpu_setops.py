@@ -202,8 +202,8 @@ def test_tile_schedules_batched_pairs(sx, schedule):
     """Tile schedules on 16 stacked full-size pairs (hundreds of tiles),
     bit-exact against the oracle: single pass 512 x 8 (-1, decoupled
     look-back), two passes (-2, count kernel, prefix kernel, write kernel),
-    single pass 128 x 16 (-3) and 64 x 32 (-4), flat tiles 512 x 8 (-5);
-    repeated launches on one scratch buffer alternate schedules.  Dilation (transforms on load, one
+    single pass 128 x 16 (-3) and 64 x 32 (-4), flat tiles 512 x 8 (-5) and
+    1024 x 8 (-6); repeated launches on one scratch buffer alternate schedules.  Dilation (transforms on load, one
     operand) runs through the same shape."""
     import torch
 
