@@ -396,27 +396,45 @@ def plain_two_level(dev, n: int = 128, steps: int = 100, regrid_every: int = 10)
             "desc": "AMR2Simulation (full-domain levels as the reference), FP64, value tag u>=0.1, moore grading 2"}
 
 
-def serve(rates, count: int, dev, seed: int = 5) -> Dict:
+def serve(rates, count: int, dev, seed: int = 5, budget_count: int = 400) -> Dict:
     """Served stream at each Poisson rate: continuous batching server vs the
-    unbatched plain loop on the same request trace."""
-    from .serve import Server, make_trace, plain_loop
+    unbatched plain loop on the same request trace (fine domains log-uniform
+    256..8192, the config-3 scale at the top).  Then a trace of config-3
+    sized jobs (fine 8192^2) arriving together, large enough that admission
+    by the 200 GiB budget binds: requests wait for memory to free up, and
+    the queueing delay (arrival to admission) is reported apart from the
+    service time (admission to result)."""
+    from .serve import HBM_BUDGET_BYTES, Server, make_trace, plain_loop
 
-    out = {"requests_per_rate": count, "mix": "70% advection (fine 256..4096 log-uniform, 10..200 steps), "
-                                             "30% set queries (rows 256..16384), HBM budget 200 GiB", "rates": {}}
+    out = {"requests_per_rate": count,
+           "mix": "70% advection (fine 256..8192 log-uniform on 4 sizes per octave, 4 disc layouts per size, "
+                  "10..200 steps), 30% set queries (rows 256..16384), HBM budget 200 GiB", "rates": {}}
     # warm-up: kernels, graph capture path, allocator
     Server(streams=4).run(make_trace(100.0, 4, seed=seed + 100, n_fine=(128, 256), steps=(4, 8), query_rows=(64, 128)))
-    for rate in rates:
-        trace = make_trace(float(rate), count, seed=seed)
+
+    def compare(trace):
         t0 = time.perf_counter()
         srv = Server(streams=4).run(trace).summary()
-        log(f"serve rate {rate}: server done in {time.perf_counter() - t0:.1f}s")
+        t1 = time.perf_counter()
         plain = plain_loop(trace).summary()
-        log(f"serve rate {rate}: plain loop done")
-        out["rates"][str(rate)] = {
-            "server": srv, "plain": plain,
+        log(f"serve: server {t1 - t0:.1f}s, plain loop {time.perf_counter() - t1:.1f}s")
+        span = max(r.arrival for r in trace)
+        return {
+            "server": srv, "plain": plain, "arrival_span_s": span,
+            "plain_falls_behind": plain["wall_s"] > 1.05 * span,
             "cell_updates_ratio": srv["cell_updates_per_s"] / max(plain["cell_updates_per_s"], 1e-9),
             "throughput_ratio": srv["throughput_req_per_s"] / max(plain["throughput_req_per_s"], 1e-9),
         }
+
+    for rate in rates:
+        out["rates"][str(rate)] = compare(make_trace(float(rate), count, seed=seed))
+    if budget_count > 0:
+        trace = make_trace(1e5, budget_count, seed=seed + 7, n_fine=(8192, 8192))
+        res = compare(trace)
+        res["requests"] = budget_count
+        res["offered_est_bytes"] = int(sum(r.est_bytes for r in trace))
+        res["budget_bytes"] = HBM_BUDGET_BYTES
+        out["budget_binding"] = res
     return out
 
 

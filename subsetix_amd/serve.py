@@ -17,7 +17,9 @@ keeps every admitted request resident and, per iteration:
     ``StackedComposite`` (one chain of set operations builds the geometry of
     all of them, one segment table per level, four launches per step for the
     whole cohort).  A cohort starts at once when the GPU has none running,
-    else when the batching window has passed or it is full.  Cohorts run on
+    else when the batching window has passed or it is full; the admitted jobs
+    are taken shortest first (by step count), so a cohort's jobs end close
+    together.  Cohorts run on
     separate streams and advance in chunks of captured-graph replays; a chunk
     stops at the next step count a job asked for, where that job's fields and
     FP64 mass are copied out.
@@ -31,6 +33,7 @@ synchronisation per step, the reference-shaped form.
 
 from __future__ import annotations
 
+import functools
 import math
 import time
 from collections import deque
@@ -75,12 +78,18 @@ def _host_set(h) -> dict:
             "begin": np.ascontiguousarray(h.begin), "end": np.ascontiguousarray(h.end)}
 
 
+@functools.lru_cache(maxsize=256)
+def _disc_layout(n_fine: int, layout: int):
+    return workloads.stencil_discs_config(n=n_fine, seed=layout)
+
+
 def advect_request(rid: int, arrival: float, *, n_fine: int, steps: int, seed: int,
-                   velocity=(1.0, 0.5), dtype=torch.float32) -> Request:
+                   velocity=(1.0, 0.5), dtype=torch.float32, layout: Optional[int] = None) -> Request:
     """A disc-refined two-level instance: fine n_fine^2 domain covered ~30% by seeded
     discs + annulus, coarse (n_fine/2)^2 full level, ratio 2, initial values
-    U[0, 1] (seeded), ``steps`` upwind + transfer steps."""
-    discs, scale, cov = workloads.stencil_discs_config(n=n_fine, seed=seed)
+    U[0, 1] (seeded), ``steps`` upwind + transfer steps.  ``layout``: the
+    disc layout's seed (default ``seed``); layouts are cached per size."""
+    discs, scale, cov = _disc_layout(int(n_fine), int(seed if layout is None else layout))
     nc = n_fine // 2
     fine_cells = 4 * int((discs.end.astype(np.int64) - discs.begin).sum())  # upper bound of prolong(restrict)
     esize = 4 if dtype == torch.float32 else 8
@@ -98,20 +107,29 @@ def query_request(rid: int, arrival: float, *, rows: int, seed: int, width: int 
     return Request(rid, "query", arrival, {"a": _host_set(a), "b": _host_set(b), "rows": rows}, est)
 
 
-def make_trace(rate: float, count: int, *, seed: int = 5, advect_frac: float = 0.7, n_fine=(256, 4096),
-               steps=(10, 200), query_rows=(256, 16384)) -> List[Request]:
+def make_trace(rate: float, count: int, *, seed: int = 5, advect_frac: float = 0.7, n_fine=(256, 8192),
+               steps=(10, 200), query_rows=(256, 16384), size_grid: int = 4, layouts: int = 4) -> List[Request]:
     """Poisson arrivals at ``rate`` requests/s; advection jobs with fine
-    domains log-uniform in ``n_fine`` (even sizes) and steps uniform in
-    ``steps``; queries with rows uniform in ``query_rows``."""
+    domains log-uniform in ``n_fine`` and steps uniform in ``steps``;
+    queries with rows uniform in ``query_rows``.  Fine sizes are rounded to
+    a log grid of ``size_grid`` points per octave (even sizes) and each job
+    takes one of ``layouts`` seeded disc layouts of its size (its initial
+    values have their own seed), so that the host builds each geometry
+    once; ``size_grid = 0`` keeps every size."""
     rng = np.random.default_rng(seed)
     t = 0.0
     out = []
+    lo, hi = np.log2(n_fine[0]), np.log2(n_fine[1])
     for rid in range(count):
         t += float(rng.exponential(1.0 / rate))
         if rng.random() < advect_frac:
-            nf = int(np.exp(rng.uniform(np.log(n_fine[0]), np.log(n_fine[1])))) // 2 * 2
+            e = rng.uniform(lo, hi)
+            if size_grid > 0:
+                e = min(max(round(e * size_grid) / size_grid, lo), hi)
+            nf = max(2, int(round(2.0 ** e)) // 2 * 2)
+            jseed = int(rng.integers(1 << 30))
             out.append(advect_request(rid, t, n_fine=nf, steps=int(rng.integers(steps[0], steps[1] + 1)),
-                                      seed=int(rng.integers(1 << 30))))
+                                      seed=jseed, layout=(1 + jseed % layouts) if layouts > 0 else None))
         else:
             out.append(query_request(rid, t, rows=int(rng.integers(query_rows[0], query_rows[1] + 1)),
                                      seed=int(rng.integers(1 << 30))))
@@ -127,6 +145,7 @@ class Scheduler:
         self.used = 0
         self.waiting: deque = deque()
         self.peak = 0
+        self.blocked = set()  # requests that found the budget full at least once
 
     def offer(self, req: Request) -> None:
         if req.est_bytes > self.budget:
@@ -139,6 +158,8 @@ class Scheduler:
             r = self.waiting.popleft()
             self.used += r.est_bytes
             out.append(r)
+        for r in self.waiting:  # every request still waiting found the budget full
+            self.blocked.add(r.rid)
         self.peak = max(self.peak, self.used)
         return out
 
@@ -368,15 +389,21 @@ class ServeStats:
     iterations: int
     peak_bytes: int
     setup_s: float = 0.0
+    blocked: int = 0
 
     def summary(self) -> dict:
         lat = {k: sorted(r.latency for r in self.results if r.kind == k) for k in ("advect", "query")}
         allat = sorted(r.latency for r in self.results)
+        queue = sorted(r.admitted - r.arrival for r in self.results)
+        service = sorted(r.finished - r.admitted for r in self.results)
 
         def pct(v, q):
             return float(np.percentile(v, q)) if v else None
 
         return {
+            "queue_delay_p50_s": pct(queue, 50), "queue_delay_p99_s": pct(queue, 99),
+            "service_p50_s": pct(service, 50), "service_p99_s": pct(service, 99),
+            "budget_blocked_requests": self.blocked,
             "requests": len(self.results), "wall_s": self.wall,
             "throughput_req_per_s": len(self.results) / self.wall,
             "cell_updates_per_s": self.cell_updates / self.wall,
@@ -391,11 +418,14 @@ class ServeStats:
 class Server:
     """Continuous batching server (see the module docstring).  ``window``:
     seconds an admitted advection job may wait for others to join its cohort
-    while other cohorts run; ``max_cohort``: instances per cohort."""
+    while other cohorts run; ``max_cohort``: instances per cohort;
+    ``max_cohort_bytes``: estimated bytes per cohort (the estimate bounds the
+    cohort's fields well under the 4 GiB a level may hold for the segment
+    stencil's 32-bit byte offsets)."""
 
     def __init__(self, *, budget_bytes: int = HBM_BUDGET_BYTES, streams: int = 4, chunk: int = 8,
                  use_graph: bool = True, device=None, keep_outputs: bool = False, window: float = 2e-3,
-                 max_cohort: int = 64):
+                 max_cohort: int = 64, max_cohort_bytes: int = 8 << 30):
         self.dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
         self.sched = Scheduler(budget_bytes)
         self.streams = [torch.cuda.Stream(self.dev) for _ in range(max(1, streams))]
@@ -404,6 +434,7 @@ class Server:
         self.keep_outputs = keep_outputs
         self.window = float(window)
         self.max_cohort = max(1, int(max_cohort))
+        self.max_cohort_bytes = int(max_cohort_bytes)
         self.qstream = torch.cuda.Stream(self.dev)
 
     def run(self, trace: List[Request]) -> ServeStats:
@@ -439,10 +470,18 @@ class Server:
                         wait_since = now()
                     waiting.append(r)
             if waiting and (not cohorts or len(waiting) >= self.max_cohort or now() - wait_since >= self.window):
+                # shortest jobs first, so that a cohort's jobs end close together (an instance that has
+                # ended still costs its cohort transfers until the whole cohort retires)
+                waiting.sort(key=lambda r: (int(r.payload["steps"]), r.arrival))
                 by_dtype: Dict[torch.dtype, List[Request]] = {}
-                for r in waiting[: self.max_cohort]:
+                take, nbytes = 0, 0
+                while (take < len(waiting) and take < self.max_cohort
+                       and (take == 0 or nbytes + waiting[take].est_bytes <= self.max_cohort_bytes)):
+                    nbytes += waiting[take].est_bytes
+                    take += 1
+                for r in waiting[:take]:
                     by_dtype.setdefault(r.payload["dtype"], []).append(r)
-                waiting = waiting[self.max_cohort:]
+                waiting = waiting[take:]
                 wait_since = now()
                 for reqs in by_dtype.values():
                     s = self.streams[rr % len(self.streams)]
@@ -499,6 +538,7 @@ class Server:
         wall = now()
         st = ServeStats(sorted(results, key=lambda r: r.rid), wall, cell_updates, qints, iters, self.sched.peak)
         st.setup_s = setup
+        st.blocked = len(self.sched.blocked)
         return st
 
 
