@@ -223,12 +223,10 @@ def cpu_batched_amr(dev, steps=10):
                                                                           threshold=(0.02, 0.02), grading=1),
                            device=dev)
     eng.run(steps)
-    cs = eng.cstart0.cpu().numpy()
-    lv0 = eng.level0_values().cpu().numpy()
     m = eng.masses().cpu().numpy()
     errs = []
     for j, o in enumerate(o64):
-        e = _err(lv0[cs[j]:cs[j + 1]], o.vals[0])
+        e = _err(eng.level_values(0, j).reshape(-1), o.vals[0])
         e.update({"n": sub[j]["n"], "mass_rel_err": float(abs(m[j] - o.mass()) / abs(o.mass()))})
         errs.append(e)
     return {"value": cells / t_all, "value_1core": c1 / t_one, "unit": "cell-updates/s", "cores": ox.num_threads(),

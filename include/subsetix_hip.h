@@ -346,8 +346,8 @@ int32_t stx_gradient(const float* data, int32_t width, int32_t height, float* ou
  * graph.  Levels are dense row-major per instance; the refine sets are bit
  * masks (one uint64 per 64-cell row chunk); work lists are chunk descriptors
  * (i << 48 | y << 24 | m) with device-side counts (d_n; NULL = bound).
- * lt* : int64[4 n] per level (side, first cell, first mask word, words per
- * row).  t: tag threshold (|grad u| >= t). */
+ * lt* : int64[5 n] per level (side, first cell, first mask word, words per
+ * row, row pitch; level 0 carries a one-cell zero ring: pitch = side + 2).  t: tag threshold (|grad u| >= t). */
 int32_t stx_amr3_dilate(const int64_t* lt, const uint64_t* list, const int32_t* d_n, int64_t bound, const uint64_t* in,
                         uint64_t* out, hipStream_t stream);
 int32_t stx_amr3_refine(const int64_t* ltp, const int64_t* ltc, const uint64_t* list, const int32_t* d_n, int64_t bound,
@@ -372,6 +372,10 @@ int32_t stx_amr3_rec_emit(int32_t kind, const int64_t* ltp, const int64_t* ltc, 
 int32_t stx_amr3_tag0_records(const int64_t* lt0, const uint64_t* list, int64_t n, uint64_t* out, hipStream_t stream);
 int32_t stx_amr3_tag0_rec(int32_t dtype, const uint64_t* rec, int64_t n, const void* v, double t, uint64_t* out,
                           hipStream_t stream);
+/* Level-0 upwind over the padded full domain of every instance (records of
+ * stx_amr3_tag0_records). */
+int32_t stx_amr3_upwind0_rec(int32_t dtype, const uint64_t* rec, int64_t n, const void* v, void* vn, const void* coef,
+                             hipStream_t stream);
 int32_t stx_amr3_stencil_rec(int32_t dtype, const uint64_t* rec1, const int32_t* d_n1, const uint64_t* rec2,
                              const int32_t* d_n2, int64_t bound, const void* v0, const void* v1, const void* v2, void* v1n,
                              void* v2n, const void* coef1, const void* coef2, hipStream_t stream);

@@ -29,14 +29,15 @@ size known only after a kernel has run:
   record; every grid and buffer is sized from the instance geometry alone
   (capacity bounds), so nothing is ever re-captured.
 
-Level 0 (the full domain of every instance) keeps the segment-table stencil
-of the interval-set engine (its geometry is fixed for the whole run); its
-gradient tags run from static records, one per 64-cell row chunk.  Every
-record kernel is a persistent grid-stride loop in which a wavefront loads 8
-or 16 records at once and issues all value loads of a group of records
-before using any, so it keeps dozens of loads in flight.  A step is four
-launches (level-0 upwind, level-1/2 upwind, two restrictions); a regrid about
-thirty, all device-side.
+Level 0 (the full domain of every instance) is stored with a one-cell ring
+of zeros around each instance, so its update and gradient tag read every
+neighbour without a test (the ring is the zero exterior); both run from
+static records, one per 64-cell row chunk.  Every record kernel is a
+persistent grid-stride loop in which a wavefront loads 8 or 16 records at
+once and issues all value loads of a group of records before using any, so
+it keeps dozens of loads in flight.  A step is four launches (level-0
+upwind, level-1/2 upwind, two restrictions); a regrid about thirty, all
+device-side.
 """
 
 from __future__ import annotations
@@ -47,13 +48,9 @@ from typing import Dict, List, Optional
 import numpy as np
 import torch
 
-from . import _core, _lib, autotune
+from . import _lib, autotune
 from ._lib import _F64, _I32, _I64, _P, check, context, ptr
 from .batch import BatchConfig
-from .expressions import IntervalSet
-from .interval_field import IntervalField, cell_offsets
-from .interval_stencil import build_stencil_plan
-from .levels import upwind_batched
 
 _lib.register("stx_amr3_dilate", _I32, [_P, _P, _P, _I64, _P, _P, _P])
 _lib.register("stx_amr3_refine", _I32, [_P, _P, _P, _P, _I64, _P, _P, _P, _P])
@@ -66,6 +63,7 @@ _lib.register("stx_amr3_rec_count", _I32, [_I32, _P, _P, _P, _P, _I64, _P, _P, _
 _lib.register("stx_amr3_rec_emit", _I32, [_I32, _P, _P, _P, _P, _I64, _P, _P, _P, _P])
 _lib.register("stx_amr3_tag0_records", _I32, [_P, _P, _I64, _P, _P])
 _lib.register("stx_amr3_tag0_rec", _I32, [_I32, _P, _I64, _P, _F64, _P, _P])
+_lib.register("stx_amr3_upwind0_rec", _I32, [_I32, _P, _I64, _P, _P, _P, _P])
 _lib.register("stx_amr3_stencil_rec", _I32, [_I32, _P, _P, _P, _P, _I64, _P, _P, _P, _P, _P, _P, _P, _P])
 _lib.register("stx_amr3_lupdate_rec", _I32, [_I32, _P, _P, _P, _P, _I64, _P, _P, _P, _P])
 _lib.register("stx_amr3_tagl_rec", _I32, [_I32, _P, _P, _P, _I64, _P, _F64, _P, _P])
@@ -79,19 +77,27 @@ def _encode(i, y, m):
 
 
 class _Level:
-    """Host and device tables of one level: side, first cell, first mask
-    word, words per row (int64[n, 4])."""
+    """Host and device tables of one level: side, cell (0, 0), first mask
+    word, words per row, row pitch (int64[n, 5]).  ``pad``: width of the ring
+    of zeros around each instance's block (level 0: 1), so a block is
+    (side + 2 pad)^2 cells; ``start`` holds the block boundaries and
+    ``ncells`` the allocation (plus 64 cells of slack after a padded level,
+    which whole-chunk loads past the last row may touch)."""
 
-    def __init__(self, sizes: np.ndarray, level: int, dev):
+    def __init__(self, sizes: np.ndarray, level: int, dev, pad: int = 0):
         self.side = (sizes << level).astype(np.int64)
+        self.pad = int(pad)
+        self.pitch = self.side + 2 * self.pad
         self.cells = self.side * self.side
         self.cw = (self.side + 63) // 64
         self.words = self.side * self.cw
-        self.cell0 = np.concatenate([[0], np.cumsum(self.cells)]).astype(np.int64)
+        self.start = np.concatenate([[0], np.cumsum(self.pitch * self.pitch)]).astype(np.int64)
+        self.origin = self.start[:-1] + self.pad * (self.pitch + 1)
         self.word0 = np.concatenate([[0], np.cumsum(self.words)]).astype(np.int64)
-        tab = np.stack([self.side, self.cell0[:-1], self.word0[:-1], self.cw], axis=1).astype(np.int64)
+        tab = np.stack([self.side, self.origin, self.word0[:-1], self.cw, self.pitch], axis=1).astype(np.int64)
         self.table = torch.from_numpy(np.ascontiguousarray(tab.reshape(-1))).to(dev)
-        self.ncells = int(self.cell0[-1])
+        self.ncells = int(self.start[-1]) + (64 if self.pad else 0)
+        self.interior = int(self.cells.sum())
         self.nwords = int(self.word0[-1])
 
     def all_chunks(self) -> np.ndarray:
@@ -125,18 +131,22 @@ class DeviceBatchedAMR:
             raise ValueError("instance sizes must be in [1, 2^22)")
         self.sizes = sizes
         dev = self.dev
-        self.L = [_Level(sizes, lv, dev) for lv in range(3)]
+        self.L = [_Level(sizes, lv, dev, pad=1 if lv == 0 else 0) for lv in range(3)]
         npdt = np.float32 if dtype == torch.float32 else np.float64
-        # per-level coefficients (a, b, dt, dx, dy) as the interval-set engine
+        # per-level coefficients (a, b, dt, dx, dy) as the interval-set engine,
         vel = np.array([i["velocity"] for i in instances], dtype=np.float64)
         finest = sizes * 4
         denom = np.abs(vel[:, 0]) * finest + np.abs(vel[:, 1]) * finest
         dts = np.where(denom > 0, self.cfg.cfl / np.maximum(denom, 1e-300), 0.0)
+        # and the correctly rounded reciprocals of dx, dy that the update's
+        # division uses (1 / d in FP64 rounds correctly to FP32 as well)
         self.coef = []
         for lv in range(3):
             nl = sizes * (1 << lv)
-            c = np.stack([vel[:, 0], vel[:, 1], dts, 1.0 / nl, 1.0 / nl], axis=1).astype(npdt).reshape(-1)
-            self.coef.append(torch.from_numpy(c).to(dev))
+            c = np.stack([vel[:, 0], vel[:, 1], dts, 1.0 / nl, 1.0 / nl], axis=1).astype(npdt)
+            rec = (1.0 / c[:, 3:5].astype(np.float64)).astype(npdt)
+            self.coef.append(torch.from_numpy(np.ascontiguousarray(np.concatenate([c, rec], axis=1).reshape(-1)))
+                             .to(dev))
         # dense level values, two buffers each
         self.V = [[torch.zeros(lv.ncells, dtype=dtype, device=dev) for _ in range(2)] for lv in self.L]
         self.cur = 0
@@ -171,10 +181,9 @@ class DeviceBatchedAMR:
         self.parity = 1  # the first regrid writes R[0], A1[0]
         check(_lib.lib().stx_amr3_tag0_records(ptr(self.L[0].table), ptr(self.l0all), self.n0, ptr(self.rec0),
                                                context(dev).stream), "tag0_records")
-        # level 0: the stacked full-domain interval set, for its segment-table stencil
-        self.cstart0 = torch.from_numpy(self.L[0].cell0).to(dev)
+        # level-0 block boundaries (the zero ring adds nothing to a mass)
+        self.cstart0 = torch.from_numpy(self.L[0].start).to(dev)
         self.cell_area0 = torch.from_numpy((1.0 / (sizes * sizes)).astype(np.float64)).to(dev)
-        self._level0_plan()
         self._init_values()
         self.step_count = 0
         self._last_regrid = 0
@@ -189,24 +198,6 @@ class DeviceBatchedAMR:
                 self._step(self.cur)
 
     # ------------------------------------------------------------------ setup
-    def _level0_plan(self) -> None:
-        gap = self.cfg.gap
-        n = self.sizes
-        base = np.zeros(self.n, np.int64)
-        base[1:] = np.cumsum(n + gap)[:-1]
-        rows = np.concatenate([np.arange(b, b + k, dtype=np.int64) for b, k in zip(base, n)]).astype(np.int32)
-        t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(self.dev)  # noqa: E731
-        self.D0 = IntervalSet._trusted(t(np.zeros(rows.size, np.int32)), t(np.repeat(n, n).astype(np.int32)),
-                                       t(np.arange(rows.size + 1, dtype=np.int32)), t(rows), dense=False)
-        self.co0 = cell_offsets(self.D0)
-        bases = t(base.astype(np.int32))
-        self.row_inst0 = _core.empty_i32(rows.size, self.dev)
-        ctx = context(self.dev)
-        check(_lib.lib().stx_row_instance(ptr(self.D0.rows), self.D0.row_count, ptr(bases), self.n,
-                                          ptr(self.row_inst0), ctx.stream), "row_instance")
-        self.inst_w0 = t(n.astype(np.int32))
-        self.plan0 = build_stencil_plan(self.D0, self.co0, 0, self.row_inst0, self.inst_w0)
-
     def _init_values(self) -> None:
         v0 = self.V[0][0]
         for k, inst in enumerate(self.instances):
@@ -216,8 +207,14 @@ class DeviceBatchedAMR:
             u = torch.zeros((n, n), dtype=torch.float64, device=self.dev)
             for cx, cy, sg, amp in inst["bumps"]:
                 u += amp * torch.exp(-((X - cx) ** 2 + (Y - cy) ** 2) / (2 * sg * sg))
-            c0 = int(self.L[0].cell0[k])
-            v0[c0:c0 + n * n].copy_(u.reshape(-1).to(self.dtype))
+            self._block(v0, 0, k).copy_(u.to(self.dtype))
+
+    def _block(self, v: torch.Tensor, level: int, i: int) -> torch.Tensor:
+        """(side, side) view of instance i's cells in a level buffer."""
+        lv = self.L[level]
+        p, pad, s = int(lv.pitch[i]), lv.pad, int(lv.side[i])
+        blk = v[int(lv.start[i]):int(lv.start[i + 1])].view(p, p)
+        return blk[pad:pad + s, pad:pad + s]
 
     # ------------------------------------------------------------------ lists
     def _scan(self, counts: torch.Tensor, d_n, bound: int, d_total) -> None:
@@ -308,8 +305,8 @@ class DeviceBatchedAMR:
         lib, st = _lib.lib(), context(self.dev).stream
         nxt = 1 - cur
         V = self.V
-        f0 = IntervalField(self.D0, V[0][cur], self.co0)
-        upwind_batched(f0, V[0][cur], V[0][nxt], self.row_inst0, self.inst_w0, self.coef[0], plan=self.plan0)
+        check(lib.stx_amr3_upwind0_rec(self.dcode, ptr(self.rec0), self.n0, ptr(V[0][cur]), ptr(V[0][nxt]),
+                                       ptr(self.coef[0]), st), "upwind0")
         cnt = self.counts
         check(lib.stx_amr3_stencil_rec(self.dcode, ptr(self.S1), ptr(cnt[3:4]), ptr(self.S2), ptr(cnt[4:5]),
                                        self.cap2 + self.cap1, ptr(V[0][cur]), ptr(V[1][cur]), ptr(V[2][cur]),
@@ -322,7 +319,7 @@ class DeviceBatchedAMR:
                                         ptr(V[0][nxt]), st), "restrict")
 
     def _count(self, steps: int) -> None:
-        check(_lib.lib().stx_amr3_accumulate(ptr(self.cell_total), ptr(self.bits), self.L[0].ncells, int(steps),
+        check(_lib.lib().stx_amr3_accumulate(ptr(self.cell_total), ptr(self.bits), self.L[0].interior, int(steps),
                                              context(self.dev).stream), "accumulate")
 
     def _steps(self, n: int) -> None:
@@ -389,7 +386,7 @@ class DeviceBatchedAMR:
 
     def cells(self) -> Dict[str, int]:
         b = self.bits.tolist()
-        return {"level0": self.L[0].ncells, "level1": 4 * b[0], "level2": 4 * b[1]}
+        return {"level0": self.L[0].interior, "level1": 4 * b[0], "level2": 4 * b[1]}
 
     def check_status(self) -> None:
         if int(self.status.item()):
@@ -400,9 +397,6 @@ class DeviceBatchedAMR:
         check(_lib.lib().stx_instance_mass(self.dcode, ptr(self.V[0][self.cur]), ptr(self.cstart0), self.n,
                                            ptr(self.cell_area0), ptr(out), context(self.dev).stream), "instance_mass")
         return out
-
-    def level0_values(self) -> torch.Tensor:
-        return self.V[0][self.cur]
 
     def level_mask(self, level: int, i: int) -> np.ndarray:
         """Dense bool (s, s) mask of instance i's level set (host copy)."""
@@ -421,7 +415,4 @@ class DeviceBatchedAMR:
     def level_values(self, level: int, i: int) -> np.ndarray:
         """Dense (s, s) values of instance i at a level (host copy; only the
         level set's cells are meaningful on levels 1 and 2)."""
-        lv = self.L[level]
-        s = int(lv.side[i])
-        v = self.V[level][self.cur]
-        return v[int(lv.cell0[i]):int(lv.cell0[i + 1])].cpu().numpy().reshape(s, s)
+        return self._block(self.V[level][self.cur], level, i).cpu().numpy()

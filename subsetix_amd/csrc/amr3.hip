@@ -3,17 +3,20 @@
 // run with no host synchronisation and can be captured once as one graph.
 //
 // Layout (per level L = 0, 1, 2; instance i has side s = n_i 2^L):
-//   * values: dense row-major per instance, cell (y, x) at cell0_i + y s + x
-//     (int64); level 0 is the full domain, levels 1 and 2 hold meaningful
-//     values only on their level sets;
+//   * values: dense row-major per instance, cell (y, x) at cell0_i + y p + x
+//     (int64; p the row pitch).  Level 0 is the full domain, stored with a
+//     one-cell ring of zeros around each instance (p = s + 2; the ring is
+//     never written, so a neighbour outside the instance reads the zero
+//     exterior); levels 1 and 2 (p = s) hold meaningful values only on their
+//     level sets;
 //   * masks (levels 0 and 1): one 64-bit word per 64-cell row chunk, word
 //     (y, m) at word0_i + y cw_i + m, bit b = cell x = 64 m + b; bits past the
 //     row end are 0;
 //   * work lists: 64-bit chunk descriptors (i << 48) | (y << 24) | m with a
 //     device-side count; consumers are persistent grid-stride kernels (one
 //     wavefront, or one lane, per chunk).
-// Level tables: int64[4 n] per level: side, first cell, first word, words
-// per row.
+// Level tables: int64[5 n] per level: side, first cell (cell (0, 0)), first
+// word, words per row, row pitch.
 //
 // The level sets follow from two refine masks: level 1 = prolong(R0),
 // level 2 = prolong(R1) (R0 at level-0 resolution, R1 at level 1).  Ghost
@@ -33,10 +36,11 @@ constexpr int64_t kMaxBlocks = 4096;
 
 struct Lvl {
     const int64_t* t;
-    __device__ __forceinline__ int32_t side(int i) const { return (int32_t)t[4 * i]; }
-    __device__ __forceinline__ int64_t cell0(int i) const { return t[4 * i + 1]; }
-    __device__ __forceinline__ int64_t word0(int i) const { return t[4 * i + 2]; }
-    __device__ __forceinline__ int32_t cw(int i) const { return (int32_t)t[4 * i + 3]; }
+    __device__ __forceinline__ int32_t side(int i) const { return (int32_t)t[5 * i]; }
+    __device__ __forceinline__ int64_t cell0(int i) const { return t[5 * i + 1]; }
+    __device__ __forceinline__ int64_t word0(int i) const { return t[5 * i + 2]; }
+    __device__ __forceinline__ int32_t cw(int i) const { return (int32_t)t[5 * i + 3]; }
+    __device__ __forceinline__ int32_t pitch(int i) const { return (int32_t)t[5 * i + 4]; }
     __device__ __forceinline__ int64_t word(int i, int y, int m) const { return word0(i) + (int64_t)y * cw(i) + m; }
 };
 
@@ -113,13 +117,52 @@ __device__ __forceinline__ bool grad_tag(T c, T l, T r, T d, T u, bool hl, bool 
     return gx2 + gy2 >= smin;
 }
 
+// Range of |a| in which div_rn's product form is exact (its residuals stay
+// normal numbers); outside it, and for 0, inf and NaN, it divides.
+template <typename T>
+struct DivRange;
+template <>
+struct DivRange<float> {
+    static constexpr float lo = 0x1p-64f, hi = 0x1p64f;
+};
+template <>
+struct DivRange<double> {
+    static constexpr double lo = 0x1p-800, hi = 0x1p800;
+};
+
+// a / d correctly rounded, for d > 0 and y = RN(1 / d) (a per-instance
+// coefficient).  q0 = a y lies within 1.5 ulp of a / d; one correction,
+// q1 = q0 + (a - q0 d) y, leaves it within 1/2 ulp + 2^-22 ulp, a faithful
+// quotient; with q1 faithful and y correctly rounded, the residual
+// a - q1 d is exact and q1 + (a - q1 d) y rounds to RN(a / d) (Markstein's
+// theorem).  Checked exhaustively in FP32 for every float a in a binade and
+// every d = 1/n, n <= 16384, at both ends of the range.
+__device__ __forceinline__ float fma_rn(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fma_rn(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+template <typename T>
+__device__ __forceinline__ T div_rn(T a, T d, T y) {
+    const T aa = a < (T)0 ? -a : a;
+    if (aa >= DivRange<T>::lo && aa <= DivRange<T>::hi) {
+        const T q0 = a * y;
+        const T r0 = fma_rn(-q0, d, a);
+        const T q1 = fma_rn(r0, y, q0);
+        const T r1 = fma_rn(-q1, d, a);
+        return fma_rn(r1, y, q1);
+    }
+    return a == (T)0 ? a : a / d;
+}
+
+// Coefficients per instance: a, b, dt, dx, dy, RN(1/dx), RN(1/dy).
+constexpr int kCoef = 7;
+
 template <typename T>
 __device__ __forceinline__ T upwind(T c, T l, T r, T d, T u, const T* k) {
     const T a = k[0], b = k[1], dt = k[2], dx = k[3], dy = k[4];
     const T nx = (a >= (T)0) ? c - l : r - c;
     const T ny = (b >= (T)0) ? c - d : u - c;
-    const T dudx = nx / dx;
-    const T dudy = ny / dy;
+    const T dudx = div_rn<T>(nx, dx, k[5]);
+    const T dudy = div_rn<T>(ny, dy, k[6]);
     const T t1 = a * dudx;
     const T t2 = b * dudy;
     return c - dt * (t1 + t2);
@@ -276,13 +319,13 @@ __global__ __launch_bounds__(kT) void popc_kernel(Lvl L, const uint64_t* __restr
 // load per record (no table or mask look-ups):
 //   stencil record (8 words), for a chunk holding cells of the level set:
 //     desc, first cell of the chunk, parent cell under its first cell,
-//     side | parent side << 32, the level-set words of the chunk, of the rows
+//     side | parent pitch << 32, the level-set words of the chunk, of the rows
 //     below and above, and the edge bits (bit 0: the cell left of the chunk,
 //     bit 1: the cell right of it);
 //   restriction record (4 words), for a parent chunk holding refined cells:
-//     first parent cell, first child cell (row 2y), child side, refine word;
+//     first parent cell, first child cell (row 2y), child pitch, refine word;
 //   level-0 record (4 words, static, one per chunk): first cell, mask word,
-//     side, chunk descriptor.
+//     side | pitch << 32, chunk descriptor.
 constexpr int kSW = 8;
 constexpr int kQW = 4;
 
@@ -327,15 +370,16 @@ __global__ __launch_bounds__(kT) void rec_emit_kernel(int kind, Lvl Lp, Lvl Lc, 
         if (rec_count(kind, Lc, ch, rw) == 0) continue;
         int64_t o = offsets[t];
         if (kind == 1) {
-            const int32_t sq = Lp.side(i), sc = Lc.side(i);
+            const int32_t pq = Lp.pitch(i), pc = Lc.pitch(i);
             uint64_t* r = out + kQW * o;
-            r[0] = (uint64_t)(Lp.cell0(i) + (int64_t)ch.y * sq + 64 * ch.m);
-            r[1] = (uint64_t)(Lc.cell0(i) + (int64_t)(2 * ch.y) * sc + 128 * ch.m);
-            r[2] = (uint64_t)sc;
+            r[0] = (uint64_t)(Lp.cell0(i) + (int64_t)ch.y * pq + 64 * ch.m);
+            r[1] = (uint64_t)(Lc.cell0(i) + (int64_t)(2 * ch.y) * pc + 128 * ch.m);
+            r[2] = (uint64_t)pc;
             r[3] = rw;
             continue;
         }
-        const int32_t s = Lc.side(i), sp = Lp.side(i);
+        // own level unpadded (pitch = side); sp: the parent level's pitch
+        const int32_t s = Lc.side(i), sp = Lp.pitch(i);
         for (int h = 0; h < 2; ++h) {
             const int M = 2 * ch.m + h;
             if (M >= Lc.cw(i) || (uint32_t)(rw >> (32 * h)) == 0u) continue;
@@ -361,65 +405,114 @@ __global__ __launch_bounds__(kT) void tag0_rec_build_kernel(Lvl L, const uint64_
                                                             uint64_t* __restrict__ out) {
     LANE_LOOP(n) {
         const Chunk ch = decode(list[t]);
-        const int32_t s = L.side(ch.i);
+        const int32_t s = L.side(ch.i), p = L.pitch(ch.i);
         uint64_t* r = out + kQW * t;
-        r[0] = (uint64_t)(L.cell0(ch.i) + (int64_t)ch.y * s + 64 * ch.m);
+        r[0] = (uint64_t)(L.cell0(ch.i) + (int64_t)ch.y * p + 64 * ch.m);
         r[1] = (uint64_t)L.word(ch.i, ch.y, ch.m);
-        r[2] = (uint64_t)s;
+        r[2] = (uint64_t)(uint32_t)s | ((uint64_t)(uint32_t)p << 32);
         r[3] = list[t];
     }
 }
 
-// Level-0 gradient tags from the static records, 16 records per wavefront
-// iteration in groups of kG: every load of a group is issued (from a safe
-// address where the cell or neighbour is absent) before any is used, so a
-// wavefront keeps 5 kG loads in flight.
-constexpr int kG = 8;
+constexpr int kG = 8;  // records whose loads a wavefront issues together
+
+// Level-0 record kernels run one block (4 wavefronts x 16 records) per tile
+// of 64 records, tiles handed to the 8 XCDs in chunks of 32 consecutive
+// tiles, so the rows above and below a chunk's rows mostly sit in the same
+// XCD's L2.
+constexpr int kXcd = 8, kXcdChunk = 32, kRecTile = 64;
+
+__device__ __forceinline__ int64_t xcd_tile(int64_t block) {
+    const int64_t xcd = block % kXcd, nth = block / kXcd;
+    return ((nth / kXcdChunk) * kXcd + xcd) * kXcdChunk + nth % kXcdChunk;
+}
+
+inline unsigned xcd_grid(int64_t n) {
+    const int64_t tiles = (n + kRecTile - 1) / kRecTile;
+    const int64_t per = (int64_t)kXcd * kXcdChunk;
+    return (unsigned)((tiles + per - 1) / per * per);
+}
 
 // Level-0 gradient tags from the static records, 16 records per wavefront
-// iteration in groups of kG: every load of a group is issued (from a safe
-// address where the cell or neighbour is absent) before any is used, so a
-// wavefront keeps 5 kG loads in flight.
+// iteration in groups of kG.  The zero ring around each instance (and the
+// slack after the last one) keeps every neighbour load in bounds, so every
+// load of a group is issued unconditionally before any is used; slots past
+// n repeat the last record (their tags are not stored).
 template <typename T>
 __global__ __launch_bounds__(kT) void tag0_rec_kernel(const uint64_t* __restrict__ rec, int64_t n, const T* __restrict__ V,
                                                       T smin, uint64_t* __restrict__ out) {
     constexpr int U = 64 / kQW;
     const int lane = lane_id();
-    const int64_t wave0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-    for (int64_t base = wave0 * U; base < n; base += nwaves * U) {
+    {
+        const int64_t base = xcd_tile(blockIdx.x) * kRecTile + (int64_t)wave_id() * U;
+        if (base >= n) return;
         const int64_t tr = base + (lane / kQW);
-        // words of records past n read as 0: side 0, no cell present
-        const uint64_t w = tr < n ? rec[tr * kQW + (lane % kQW)] : 0ull;
+        const uint64_t w = rec[(tr < n ? tr : n - 1) * kQW + (lane % kQW)];
 #pragma unroll
         for (int g = 0; g < U; g += kG) {
             T cv[kG], l[kG], r[kG], d[kG], up[kG];
 #pragma unroll
             for (int u = 0; u < kG; ++u) {
                 const int k = g + u;
-                const int64_t c0 = (int64_t)bcast(w, kQW * k);
-                const int32_t s = (int32_t)bcast(w, kQW * k + 2);
-                const Chunk ch = decode(bcast(w, kQW * k + 3));
-                const int32_t y = ch.y, x = 64 * ch.m + lane;
-                const bool act = x < s;
-                const int64_t c = act ? c0 + lane : c0;
+                const int64_t c = (int64_t)bcast(w, kQW * k) + lane;
+                const int64_t p = (int64_t)(bcast(w, kQW * k + 2) >> 32);
                 cv[u] = V[c];
-                l[u] = V[act && x > 0 ? c - 1 : c];
-                r[u] = V[act && x + 1 < s ? c + 1 : c];
-                d[u] = V[act && y > 0 ? c - s : c];
-                up[u] = V[act && y + 1 < s ? c + s : c];
+                l[u] = V[c - 1];
+                r[u] = V[c + 1];
+                d[u] = V[c - p];
+                up[u] = V[c + p];
             }
 #pragma unroll
             for (int u = 0; u < kG; ++u) {
                 const int k = g + u;
                 const int64_t widx = (int64_t)bcast(w, kQW * k + 1);
-                const int32_t s = (int32_t)bcast(w, kQW * k + 2);
+                const int32_t s = (int32_t)(uint32_t)bcast(w, kQW * k + 2);
                 const Chunk ch = decode(bcast(w, kQW * k + 3));
                 const int32_t y = ch.y, x = 64 * ch.m + lane;
                 const bool tag = x < s && grad_tag<T>(cv[u], l[u], r[u], d[u], up[u], x > 0, x + 1 < s, y > 0,
                                                       y + 1 < s, smin);
                 const uint64_t bits = __ballot(tag);
                 if (lane == 0 && base + k < n) out[widx] = bits;
+            }
+        }
+    }
+}
+
+// Level-0 upwind over the padded full domain from the same records: the
+// ring supplies the zero exterior, so the update needs no neighbour test.
+template <typename T>
+__global__ __launch_bounds__(kT) void upwind0_rec_kernel(const uint64_t* __restrict__ rec, int64_t n,
+                                                         const T* __restrict__ V, T* __restrict__ Vn,
+                                                         const T* __restrict__ coef) {
+    constexpr int U = 64 / kQW;
+    const int lane = lane_id();
+    {
+        const int64_t base = xcd_tile(blockIdx.x) * kRecTile + (int64_t)wave_id() * U;
+        if (base >= n) return;
+        const int64_t tr = base + (lane / kQW);
+        const uint64_t w = rec[(tr < n ? tr : n - 1) * kQW + (lane % kQW)];
+#pragma unroll
+        for (int g = 0; g < U; g += kG) {
+            T cv[kG], l[kG], r[kG], d[kG], up[kG];
+#pragma unroll
+            for (int u = 0; u < kG; ++u) {
+                const int k = g + u;
+                const int64_t c = (int64_t)bcast(w, kQW * k) + lane;
+                const int64_t p = (int64_t)(bcast(w, kQW * k + 2) >> 32);
+                cv[u] = V[c];
+                l[u] = V[c - 1];
+                r[u] = V[c + 1];
+                d[u] = V[c - p];
+                up[u] = V[c + p];
+            }
+#pragma unroll
+            for (int u = 0; u < kG; ++u) {
+                const int k = g + u;
+                const int32_t s = (int32_t)(uint32_t)bcast(w, kQW * k + 2);
+                const Chunk ch = decode(bcast(w, kQW * k + 3));
+                if (base + k >= n || 64 * ch.m + lane >= s) continue;
+                const int64_t c = (int64_t)bcast(w, kQW * k) + lane;
+                Vn[c] = upwind<T>(cv[u], l[u], r[u], d[u], up[u], coef + kCoef * ch.i);
             }
         }
     }
@@ -492,7 +585,7 @@ __device__ __forceinline__ void stencil_records(const uint64_t* __restrict__ rec
             const T rv = x + 1 >= r.s ? (T)0 : rr[u];
             const T dv = r.y == 0 ? (T)0 : d[u];
             const T uv = r.y + 1 >= r.s ? (T)0 : up[u];
-            Vn[r.c0 + lane] = upwind<T>(cv[u], lv, rv, dv, uv, coef + 5 * r.i);
+            Vn[r.c0 + lane] = upwind<T>(cv[u], lv, rv, dv, uv, coef + kCoef * r.i);
         }
     }
 }
@@ -709,7 +802,7 @@ int32_t stx_amr3_tag0_records(const int64_t* lt0, const uint64_t* list, int64_t 
 int32_t stx_amr3_tag0_rec(int32_t dtype, const uint64_t* rec, int64_t n, const void* v, double t, uint64_t* out,
                           hipStream_t stream) {
     if (n <= 0) return STX_OK;
-    const unsigned g = grid_for((n + 15) / 16, kT / 64);
+    const unsigned g = xcd_grid(n);
     if (dtype == 4)
         hipLaunchKernelGGL(tag0_rec_kernel<float>, dim3(g), dim3(kT), 0, stream, rec, n, (const float*)v,
                            sqrt_threshold<float, uint32_t>((float)t), out);
@@ -785,6 +878,21 @@ int32_t stx_amr3_restrict_rec(int32_t dtype, int32_t acc64, const uint64_t* rec,
     else
         return stx_fail(STX_EINVAL, "amr3_restrict_rec: dtype must be 4 or 8");
     return stx_launch_status("amr3_restrict_rec");
+}
+
+int32_t stx_amr3_upwind0_rec(int32_t dtype, const uint64_t* rec, int64_t n, const void* v, void* vn, const void* coef,
+                             hipStream_t stream) {
+    if (n <= 0) return STX_OK;
+    const unsigned g = xcd_grid(n);
+    if (dtype == 4)
+        hipLaunchKernelGGL(upwind0_rec_kernel<float>, dim3(g), dim3(kT), 0, stream, rec, n, (const float*)v, (float*)vn,
+                           (const float*)coef);
+    else if (dtype == 8)
+        hipLaunchKernelGGL(upwind0_rec_kernel<double>, dim3(g), dim3(kT), 0, stream, rec, n, (const double*)v,
+                           (double*)vn, (const double*)coef);
+    else
+        return stx_fail(STX_EINVAL, "amr3_upwind0_rec: dtype must be 4 or 8");
+    return stx_launch_status("amr3_upwind0_rec");
 }
 
 }  // extern "C"

@@ -48,11 +48,9 @@ def _compare(eng, j, o, what):
 
 
 def _oracles(eng, insts, dtype, cfg, idx):
-    cs = eng.cstart0.cpu().numpy()
-    v0 = eng.level0_values().cpu().numpy()
-    return [ox.Instance3(int(insts[i]["n"]), v0[cs[i]:cs[i + 1]], insts[i]["velocity"], dtype=dtype, levels=3,
-                         cfl=cfg.cfl, threshold=cfg.threshold, grading=cfg.grading, regrid_every=cfg.regrid_every,
-                         accumulate=cfg.accumulate) for i in idx]
+    return [ox.Instance3(int(insts[i]["n"]), eng.level_values(0, i).reshape(-1), insts[i]["velocity"], dtype=dtype,
+                         levels=3, cfl=cfg.cfl, threshold=cfg.threshold, grading=cfg.grading,
+                         regrid_every=cfg.regrid_every, accumulate=cfg.accumulate) for i in idx]
 
 
 @pytest.mark.parametrize("dt", ["f32", "f64"])
