@@ -272,10 +272,17 @@ class BatchedAMR:
         self._gstream.wait_stream(torch.cuda.current_stream(self.dev))
         with torch.cuda.stream(self._gstream):
             context(self.dev)  # per-stream context allocated outside the capture
+        # capture_begin / capture_end on the side stream: the graph context
+        # manager would synchronise the device and empty the allocator cache
+        # at every capture, and a capture follows every regrid
         g = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(g, stream=self._gstream):
-            self._step_kernels(start)
-            self._step_kernels(1 - start)
+        with torch.cuda.stream(self._gstream):
+            g.capture_begin()
+            try:
+                self._step_kernels(start)
+                self._step_kernels(1 - start)
+            finally:
+                g.capture_end()
         torch.cuda.current_stream(self.dev).wait_stream(self._gstream)
         return g
 

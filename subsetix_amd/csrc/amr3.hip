@@ -433,6 +433,28 @@ inline unsigned xcd_grid(int64_t n) {
     return (unsigned)((tiles + per - 1) / per * per);
 }
 
+// Persistent record kernels whose count is known only on the device: the
+// records split into one contiguous slab per XCD (a multiple of `per_wave`
+// records each), walked by that XCD's wavefronts (blocks b with b % 8 the
+// XCD) in steps of `per_wave` records; the grid is a multiple of 8 blocks.
+struct Slab {
+    int64_t begin, end, step;
+};
+
+__device__ __forceinline__ Slab xcd_slab(int64_t n, int per_wave) {
+    const int64_t xcd = blockIdx.x % kXcd, j = blockIdx.x / kXcd;
+    const int64_t wpb = blockDim.x >> 6;
+    const int64_t waves = (int64_t)(gridDim.x / kXcd) * wpb;
+    const int64_t per = ((n + kXcd - 1) / kXcd + per_wave - 1) / per_wave * per_wave;
+    const int64_t end = (xcd + 1) * per;
+    return Slab{xcd * per + (j * wpb + wave_id()) * per_wave, end < n ? end : n, waves * per_wave};
+}
+
+inline unsigned slab_grid(int64_t items, int per_block) {
+    const unsigned g = grid_for(items, per_block);
+    return (g + kXcd - 1) / kXcd * kXcd;
+}
+
 // Level-0 gradient tags from the static records, 16 records per wavefront
 // iteration in groups of kG.  The zero ring around each instance (and the
 // slack after the last one) keeps every neighbour load in bounds, so every
@@ -547,9 +569,10 @@ __device__ __forceinline__ SRec srec(uint64_t w, int u) {
 template <typename T>
 __device__ __forceinline__ void stencil_records(const uint64_t* __restrict__ rec, int64_t n, const T* __restrict__ Vp,
                                                 const T* __restrict__ Vc, T* __restrict__ Vn, const T* __restrict__ coef,
-                                                int lane, int64_t wave0, int64_t nwaves) {
+                                                int lane) {
     constexpr int U = 64 / kSW;
-    for (int64_t base = wave0 * U; base < n; base += nwaves * U) {
+    const Slab sl = xcd_slab(n, U);
+    for (int64_t base = sl.begin; base < sl.end; base += sl.step) {
         const int64_t tr = base + (lane / kSW);
         // words of records past n read as 0: no cell of the level set
         const uint64_t w = tr < n ? rec[tr * kSW + (lane % kSW)] : 0ull;
@@ -598,10 +621,8 @@ __global__ __launch_bounds__(kT) void stencil_rec_kernel(const uint64_t* __restr
                                                          T* __restrict__ V2n, const T* __restrict__ coef1,
                                                          const T* __restrict__ coef2) {
     const int lane = lane_id();
-    const int64_t wave0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-    stencil_records<T>(rec1, *d_n1, V0, V1, V1n, coef1, lane, wave0, nwaves);
-    stencil_records<T>(rec2, *d_n2, V1, V2, V2n, coef2, lane, wave0, nwaves);
+    stencil_records<T>(rec1, *d_n1, V0, V1, V1n, coef1, lane);
+    stencil_records<T>(rec2, *d_n2, V1, V2, V2n, coef2, lane);
 }
 
 // Level-set values after a regrid from stencil records (new level set):
@@ -613,9 +634,8 @@ __global__ __launch_bounds__(kT) void lupdate_rec_kernel(Lvl Lp, Lvl Lc, const u
     constexpr int U = 64 / kSW;
     const int64_t n = *d_n;
     const int lane = lane_id();
-    const int64_t wave0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-    for (int64_t base = wave0 * U; base < n; base += nwaves * U) {
+    const Slab sl = xcd_slab(n, U);
+    for (int64_t base = sl.begin; base < sl.end; base += sl.step) {
         const int64_t tr = base + (lane / kSW);
         const uint64_t w = tr < n ? rec[tr * kSW + (lane % kSW)] : 0ull;
 #pragma unroll
@@ -635,9 +655,8 @@ __global__ __launch_bounds__(kT) void tagl_rec_kernel(Lvl Lc, const uint64_t* __
     constexpr int U = 64 / kSW;
     const int64_t n = *d_n;
     const int lane = lane_id();
-    const int64_t wave0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-    for (int64_t base = wave0 * U; base < n; base += nwaves * U) {
+    const Slab sl = xcd_slab(n, U);
+    for (int64_t base = sl.begin; base < sl.end; base += sl.step) {
         const int64_t tr = base + (lane / kSW);
         const uint64_t w = tr < n ? rec[tr * kSW + (lane % kSW)] : 0ull;
 #pragma unroll
@@ -672,9 +691,8 @@ __global__ __launch_bounds__(kT) void restrict_rec_kernel(const uint64_t* __rest
     constexpr int U = 64 / kQW;
     const int64_t n = *d_n;
     const int lane = lane_id();
-    const int64_t wave0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-    for (int64_t base = wave0 * U; base < n; base += nwaves * U) {
+    const Slab sl = xcd_slab(n, U);
+    for (int64_t base = sl.begin; base < sl.end; base += sl.step) {
         const int64_t tr = base + (lane / kQW);
         const uint64_t w = tr < n ? rec[tr * kQW + (lane % kQW)] : 0ull;
 #pragma unroll
@@ -818,7 +836,7 @@ int32_t stx_amr3_stencil_rec(int32_t dtype, const uint64_t* rec1, const int32_t*
                              const int32_t* d_n2, int64_t bound, const void* v0, const void* v1, const void* v2, void* v1n,
                              void* v2n, const void* coef1, const void* coef2, hipStream_t stream) {
     if (bound <= 0) return STX_OK;
-    const unsigned g = grid_for((bound + 7) / 8, kT / 64);
+    const unsigned g = slab_grid((bound + 7) / 8, kT / 64);
     if (dtype == 4)
         hipLaunchKernelGGL(stencil_rec_kernel<float>, dim3(g), dim3(kT), 0, stream, rec1, d_n1, rec2, d_n2,
                            (const float*)v0, (const float*)v1, (const float*)v2, (float*)v1n, (float*)v2n,
@@ -835,7 +853,7 @@ int32_t stx_amr3_stencil_rec(int32_t dtype, const uint64_t* rec1, const int32_t*
 int32_t stx_amr3_lupdate_rec(int32_t dtype, const int64_t* ltp, const int64_t* ltc, const uint64_t* rec, const int32_t* d_n,
                              int64_t bound, const uint64_t* rold, const void* vp, void* vc, hipStream_t stream) {
     if (bound <= 0) return STX_OK;
-    const unsigned g = grid_for((bound + 7) / 8, kT / 64);
+    const unsigned g = slab_grid((bound + 7) / 8, kT / 64);
     if (dtype == 4)
         hipLaunchKernelGGL(lupdate_rec_kernel<float>, dim3(g), dim3(kT), 0, stream, Lvl{ltp}, Lvl{ltc}, rec, d_n, rold,
                            (const float*)vp, (float*)vc);
@@ -850,7 +868,7 @@ int32_t stx_amr3_lupdate_rec(int32_t dtype, const int64_t* ltp, const int64_t* l
 int32_t stx_amr3_tagl_rec(int32_t dtype, const int64_t* ltc, const uint64_t* rec, const int32_t* d_n, int64_t bound,
                           const void* v, double t, uint64_t* out, hipStream_t stream) {
     if (bound <= 0) return STX_OK;
-    const unsigned g = grid_for((bound + 7) / 8, kT / 64);
+    const unsigned g = slab_grid((bound + 7) / 8, kT / 64);
     if (dtype == 4)
         hipLaunchKernelGGL(tagl_rec_kernel<float>, dim3(g), dim3(kT), 0, stream, Lvl{ltc}, rec, d_n, (const float*)v,
                            sqrt_threshold<float, uint32_t>((float)t), out);
@@ -865,7 +883,7 @@ int32_t stx_amr3_tagl_rec(int32_t dtype, const int64_t* ltc, const uint64_t* rec
 int32_t stx_amr3_restrict_rec(int32_t dtype, int32_t acc64, const uint64_t* rec, const int32_t* d_n, int64_t bound,
                               const void* vc, void* vp, hipStream_t stream) {
     if (bound <= 0) return STX_OK;
-    const unsigned g = grid_for((bound + 15) / 16, kT / 64);
+    const unsigned g = slab_grid((bound + 15) / 16, kT / 64);
     if (dtype == 4 && acc64)
         hipLaunchKernelGGL((restrict_rec_kernel<float, double>), dim3(g), dim3(kT), 0, stream, rec, d_n, (const float*)vc,
                            (float*)vp, 0.25);
