@@ -285,9 +285,16 @@ class Cohort:
         return self.comp.cells_per_step(j) * self.steps[j]
 
 
+def _query_intervals(r: Request) -> int:
+    return int(r.payload["a"]["off"][-1]) + int(r.payload["b"]["off"][-1])
+
+
 def _stack_queries(reqs: List[Request], dev):
     """Row-stack the A and B sets of several queries (16 empty rows between
-    queries); returns device sets and each query's row range."""
+    queries); returns device sets and each query's row range.  The stacked
+    arrays are written once into pinned host buffers and copied to the
+    device asynchronously on the current stream, so the host goes on at once
+    (a copy from pageable memory would hold it for the whole transfer)."""
     gap = 16
     shift = 0
     parts_a, parts_b, ranges = [], [], []
@@ -299,15 +306,23 @@ def _stack_queries(reqs: List[Request], dev):
         shift += r.payload["rows"] + gap
 
     def cat(parts):
-        rows = np.concatenate([h["rows"] + s for h, s in parts]).astype(np.int32)
-        beg = np.concatenate([h["begin"] for h, _ in parts])
-        end = np.concatenate([h["end"] for h, _ in parts])
-        offs, base = [np.zeros(1, np.int64)], 0
-        for h, _ in parts:
-            offs.append(h["off"][1:].astype(np.int64) + base)
-            base += int(h["off"][-1])
-        off = np.concatenate(offs).astype(np.int32)
-        return _device_set({"rows": rows, "off": off, "begin": beg, "end": end}, dev)
+        nrows = sum(h["rows"].size for h, _ in parts)
+        nint = sum(int(h["off"][-1]) for h, _ in parts)
+        host = {k: torch.empty(n, dtype=torch.int32, pin_memory=True)
+                for k, n in (("rows", nrows), ("off", nrows + 1), ("begin", nint), ("end", nint))}
+        hv = {k: t.numpy() for k, t in host.items()}
+        hv["off"][0] = 0
+        r0, i0 = 0, 0
+        for h, s in parts:
+            nr, ni = h["rows"].size, int(h["off"][-1])
+            np.add(h["rows"], s, out=hv["rows"][r0:r0 + nr], casting="unsafe")
+            np.add(h["off"][1:], i0, out=hv["off"][r0 + 1:r0 + nr + 1], casting="unsafe")
+            hv["begin"][i0:i0 + ni] = h["begin"]
+            hv["end"][i0:i0 + ni] = h["end"]
+            r0 += nr
+            i0 += ni
+        d = {k: t.to(dev, non_blocking=True) for k, t in host.items()}
+        return IntervalSet._trusted(d["begin"], d["end"], d["off"], d["rows"], dense=None)
 
     return cat(parts_a), cat(parts_b), ranges
 
@@ -421,11 +436,12 @@ class Server:
     while other cohorts run; ``max_cohort``: instances per cohort;
     ``max_cohort_bytes``: estimated bytes per cohort (the estimate bounds the
     cohort's fields well under the 4 GiB a level may hold for the segment
-    stencil's 32-bit byte offsets)."""
+    stencil's 32-bit byte offsets); ``query_batch``: input intervals per
+    query batch."""
 
     def __init__(self, *, budget_bytes: int = HBM_BUDGET_BYTES, streams: int = 4, chunk: int = 8,
                  use_graph: bool = True, device=None, keep_outputs: bool = False, window: float = 2e-3,
-                 max_cohort: int = 64, max_cohort_bytes: int = 8 << 30):
+                 max_cohort: int = 64, max_cohort_bytes: int = 8 << 30, query_batch: int = 1 << 21):
         self.dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
         self.sched = Scheduler(budget_bytes)
         self.streams = [torch.cuda.Stream(self.dev) for _ in range(max(1, streams))]
@@ -435,6 +451,7 @@ class Server:
         self.window = float(window)
         self.max_cohort = max(1, int(max_cohort))
         self.max_cohort_bytes = int(max_cohort_bytes)
+        self.query_batch = int(query_batch)
         self.qstream = torch.cuda.Stream(self.dev)
 
     def run(self, trace: List[Request]) -> ServeStats:
@@ -442,6 +459,7 @@ class Server:
         cohorts: List[Cohort] = []
         qbatches: List[QueryBatch] = []
         waiting: List[Request] = []  # admitted advection jobs without a cohort yet
+        qwait: deque = deque()  # admitted queries not yet in a batch
         wait_since = 0.0
         admitted_at: Dict[int, float] = {}
         results: List[Result] = []
@@ -455,16 +473,15 @@ class Server:
         t0 = time.perf_counter()
         now = lambda: time.perf_counter() - t0
 
-        while pending or cohorts or qbatches or waiting or self.sched.waiting:
+        while pending or cohorts or qbatches or waiting or qwait or self.sched.waiting:
             t = now()
             while pending and pending[0].arrival <= t:
                 self.sched.offer(pending.popleft())
-            queries = []
             progressed = False
             for r in self.sched.admit():
                 admitted_at[r.rid] = now()
                 if r.kind == "query":
-                    queries.append(r)
+                    qwait.append(r)
                 else:
                     if not waiting:
                         wait_since = now()
@@ -512,8 +529,14 @@ class Server:
                     still.append(c)
                 progressed = True
             cohorts = still
-            # queries after the cohorts: their host work never delays a cohort
-            if queries:
+            # queries after the cohorts, at most query_batch intervals per loop
+            # iteration: a batch's host work (stacking, copies) stays short,
+            # so the cohorts' next chunks are never held back for long
+            if qwait:
+                queries, nint = [], 0
+                while qwait and (not queries or nint + _query_intervals(qwait[0]) <= self.query_batch):
+                    nint += _query_intervals(qwait[0])
+                    queries.append(qwait.popleft())
                 iters += 1
                 self.qstream.wait_stream(main)
                 qbatches.append(QueryBatch(queries, self.dev, self.qstream))
