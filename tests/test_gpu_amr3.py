@@ -141,3 +141,35 @@ def test_device_regrid_config_batch_vs_oracle(torch_mod):
     for i, o in zip(picks, oracles):
         assert abs(m[i] - o.mass()) <= 1e-10 * abs(o.mass())
     eng.check_status()
+
+
+@pytest.mark.parametrize("grading,accumulate", [(2, "f64"), (1, "f32")])
+def test_device_regrid_edge_sizes_vs_oracle(torch_mod, grading, accumulate):
+    """Sizes at the edges of the chunk layout (1, 2, 3, 31, 32, 33, 63, 64,
+    65, 127 coarse cells: level rows of 1..508 cells, partial and exact
+    64-cell chunks), a zero velocity component, grading 2 and FP32
+    restriction sums, against the oracle's instance engine."""
+    from subsetix_amd import workloads
+    from subsetix_amd.amr3 import DeviceBatchedAMR
+    from subsetix_amd.batch import BatchConfig
+
+    base = workloads.batched_instances(count=10, seed=33, nmin=16, nmax=64)
+    sizes = [1, 2, 3, 31, 32, 33, 63, 64, 65, 127]
+    insts = [dict(b, n=n) for b, n in zip(base, sizes)]
+    insts[3] = dict(insts[3], velocity=(0.0, insts[3]["velocity"][1]))
+    insts[7] = dict(insts[7], velocity=(insts[7]["velocity"][0], 0.0))
+    cfg = BatchConfig(levels=3, regrid_every=3, threshold=(0.02, 0.02), grading=grading, accumulate=accumulate,
+                      use_graph=True)
+    eng = DeviceBatchedAMR(insts, config=cfg)
+    oracles = _oracles(eng, insts, np.float32, cfg, range(len(insts)))
+    for steps in (2, 5, 4):
+        print("edge sizes", grading, accumulate, "steps", steps, flush=True)
+        eng.run(steps)
+        for o in oracles:
+            o.run(steps)
+        for j, o in enumerate(oracles):
+            _compare(eng, j, o, f"grading {grading} {accumulate} instance {j} (n {sizes[j]}) step {eng.step_count}")
+    m = eng.masses().cpu().numpy()
+    for j, o in enumerate(oracles):
+        assert abs(m[j] - o.mass()) <= 1e-10 * max(abs(o.mass()), 1e-300)
+    eng.check_status()
