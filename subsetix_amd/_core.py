@@ -143,7 +143,9 @@ def setop_raw(op: int, a, b, *, nrows: int, d_nrows: Optional[torch.Tensor] = No
     g = group or _GROUP_OVERRIDE or 0
     if g == 0:
         tile_ok = map_a is None and map_b is None and d_nrows is None
-        cands = (((-6, -5, -1, -2, -3, -4) if b is not None else (-6, -5, -1, -3, -4)) if tile_ok else ()) + autotune.GROUPS
+        plain2 = b is not None and xf_a[0] == XF_NONE and xf_b[0] == XF_NONE
+        cands = (((-7,) if plain2 else ()) + ((-6, -5, -1, -2, -3, -4) if b is not None else (-6, -5, -1, -3, -4))
+                 if tile_ok else ()) + autotune.GROUPS
         key = ("setop", int(op), tile_ok, autotune.bucket(na + nb), autotune.bucket((na + nb) / max(nrows, 1)))
         g = autotune.pick(key, lambda: {c: (lambda c=c: launch(c)) for c in cands}) or 0
         if g == 0 and not tile_ok:

@@ -11,7 +11,9 @@ their mean per row), so one tuning serves a regrid cadence or a stream of
 similar queries.
 
 Families and candidates:
-  setop      tile kernel 512 x 8 single pass (-1) or two passes (-2),
+  setop      flat tiles 1024 x 8, register-output form (-7, two operands
+             read as stored), flat tiles 1024 x 8 (-6) and 512 x 8 (-5),
+             tile kernel 512 x 8 single pass (-1) or two passes (-2),
              128 x 16 (-3), 64 x 32 (-4) (threads x positions per thread),
              one lane group of 4 / 16 / 64 lanes per row (4 / 16 / 64);
              operands with row maps have the lane-group candidates only

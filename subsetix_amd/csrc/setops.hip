@@ -1037,6 +1037,329 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(tile_waves_p
     flat_store<OP, TB>(ft, A, B, U, (int64_t)s_excl, s_k, out_off, ob, oe, capacity, d_total, status);
 }
 
+// ---------------------------------------------------------------------------
+// Flat tile kernel, register-output form (identity row maps, operands read as
+// stored, two operands).  The same keyed merge as setop_flat_kernel, with
+// less issue work per interval:
+//   * staging: thread t loads 8 consecutive intervals of ONE operand (A
+//     chunks first, then B chunks), so its 8 LDS slots are contiguous (A and
+//     B start on slots that are multiples of 8) and one row lookup serves the
+//     chunk;
+//   * the walk keeps each position's begin key and, for union, the largest
+//     end key before it (lm); a position starts an output run when its begin
+//     is above both lm and the block carry;
+//   * the block scans combine the per-wave values lane-parallel (one LDS
+//     read and a 16-lane DPP scan per thread);
+//   * once every walk is done the staged keys are dead: each emitted
+//     position writes its decoded interval straight into the compacted
+//     output (slot = the position's rank among the tile's emitted
+//     positions), and after the look-back one coalesced copy moves the
+//     tile's outputs to HBM;
+//   * row offsets: row r of the tile starts at merged position M[r] = (A
+//     intervals before r) + (B intervals before r), and its offset is the
+//     number of outputs emitted before M[r]: the exclusive count of the
+//     thread owning M[r] plus the emitted positions below M[r] in its mask.
+// A tile that this layout cannot hold goes through setop_flat_kernel's tile
+// code (flat or one wavefront per row) in the same launch.
+constexpr int kFlat2Base = 8;  // first A slot; A's lower sentinel sits at slot 7
+
+// Runs one tile through setop_flat_kernel's tile code (the flat layout of
+// that kernel, or one wavefront per row), look-back and store included.
+template <int OP, int TB, int VT>
+__device__ __forceinline__ void flat_tile_whole(const Operand& A, const Operand& B, int64_t U, int32_t RPT, uint32_t tile,
+                                                int2* s_k, FlatShared<TB>& sh, LookbackScratch lb, uint32_t epoch,
+                                                int32_t* out_off, int32_t* ob, int32_t* oe, int64_t capacity,
+                                                int32_t* d_total, uint32_t* status, int64_t spin_limit,
+                                                uint32_t* s_excl) {
+    const FlatTile ft = flat_produce<OP, false, TB, VT>(A, B, U, RPT, tile, s_k, sh, lb, epoch);
+    if (wave_id() == 0) {
+        const uint32_t e = lookback_exclusive<1, true>(lb, tile, (uint32_t)ft.total, epoch, status, spin_limit);
+        if (lane_id() == 0) *s_excl = e;
+    }
+    __syncthreads();
+    flat_store<OP, TB>(ft, A, B, U, (int64_t)*s_excl, s_k, out_off, ob, oe, capacity, d_total, status);
+}
+
+template <int OP, int TB, int VT>
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(tile_waves_per_eu<VT>()))) void setop_flat2_kernel(
+    Operand A, Operand B, int64_t U, int32_t RPT, uint32_t ntiles, int32_t* __restrict__ out_off,
+    int32_t* __restrict__ ob, int32_t* __restrict__ oe, int64_t capacity, int32_t* __restrict__ d_total,
+    uint32_t* status, void* scratch, uint32_t epoch, int64_t spin_limit) {
+    static_assert(VT == 8, "one 8-interval chunk per thread");
+    constexpr int kTW = TB / STX_WAVE;
+    static_assert(kTW <= 16, "the range and scan combines read one 16-lane row");
+    constexpr int kPCAP = TB * VT;
+    constexpr int kMaxRows = flat_rows_per_tile<TB>();
+    constexpr int32_t kHiY = OP == 2 ? INT32_MAX : INT32_MIN;
+    __shared__ __align__(16) int2 s_k[sk_size(kPCAP)];
+    // tile-local row starts of A at [0, nr] and of B at [kMaxRows + 2, ...];
+    // the same memory is setop_flat_kernel's FlatShared for tiles it takes
+    __shared__ __align__(16) int32_t s_aux[sizeof(FlatShared<TB>) / 4 > 2 * kMaxRows + 4 ? sizeof(FlatShared<TB>) / 4
+                                                                                         : 2 * kMaxRows + 4];
+    __shared__ int32_t s_red[2 * kTW];
+    __shared__ int32_t s_wmax[kTW];
+    __shared__ int32_t s_wsum[kTW];
+    __shared__ uint32_t s_excl;
+    LookbackScratch lb = lookback_scratch(scratch, ntiles);
+    const uint32_t tile = blockIdx.x;
+    const int tid = threadIdx.x, lane = lane_id(), wave = wave_id();
+    const int64_t row0 = (int64_t)tile * RPT;
+    const int nr = (int)(U - row0 < RPT ? (U - row0 > 0 ? U - row0 : 0) : RPT);
+    const int32_t a_base = A.off[row0];
+    const int32_t b_base = B.off[row0];
+    const int32_t nA = A.off[row0 + nr] - a_base;
+    const int32_t nB = B.off[row0 + nr] - b_base;
+    const int32_t P = nA + nB;
+    constexpr int32_t a0 = kFlat2Base;
+    const int32_t b0 = (a0 + nA + VT + 2 + 7) & ~7;  // past A's VT + 1 upper sentinels and B's lower one
+    if (!(b0 + nB + VT + 1 <= kPCAP && nr <= kMaxRows)) {  // uniform: the tile does not fit this layout
+        flat_tile_whole<OP, TB, VT>(A, B, U, RPT, tile, s_k, *reinterpret_cast<FlatShared<TB>*>(s_aux), lb, epoch,
+                                    out_off, ob, oe, capacity, d_total, status, spin_limit, &s_excl);
+        return;
+    }
+    int32_t* s_ra = s_aux;
+    int32_t* s_rb = s_aux + kMaxRows + 2;
+    for (int r = tid; r <= nr; r += TB) {
+        s_ra[r] = A.off[row0 + r] - a_base;
+        s_rb[r] = B.off[row0 + r] - b_base;
+    }
+    if (tid == 0) {
+        s_ra[nr + 1] = INT32_MAX;
+        s_rb[nr + 1] = INT32_MAX;
+    }
+    // this thread's chunk: 8 consecutive intervals of A (chunks [0, ca)) or B
+    const int32_t ca = (nA + 7) >> 3;
+    const bool inA = tid < ca;
+    const int32_t c = inA ? tid : tid - ca;
+    const int32_t q0 = c << 3;
+    const int32_t nq = inA ? nA : nB;
+    const int32_t n_mine = nq - q0 < 0 ? 0 : (nq - q0 > 8 ? 8 : nq - q0);  // 0 past the B chunks
+    const int32_t* pb = inA ? A.beg + a_base + q0 : B.beg + b_base + q0;
+    const int32_t* pe = inA ? A.end + a_base + q0 : B.end + b_base + q0;
+    int32_t xb[VT], xe[VT];
+    if (n_mine == 8) {
+        int4 v0, v1, w0, w1;
+        __builtin_memcpy(&v0, __builtin_assume_aligned(pb, 4), 16);
+        __builtin_memcpy(&v1, __builtin_assume_aligned(pb + 4, 4), 16);
+        __builtin_memcpy(&w0, __builtin_assume_aligned(pe, 4), 16);
+        __builtin_memcpy(&w1, __builtin_assume_aligned(pe + 4, 4), 16);
+        xb[0] = v0.x, xb[1] = v0.y, xb[2] = v0.z, xb[3] = v0.w, xb[4] = v1.x, xb[5] = v1.y, xb[6] = v1.z, xb[7] = v1.w;
+        xe[0] = w0.x, xe[1] = w0.y, xe[2] = w0.z, xe[3] = w0.w, xe[4] = w1.x, xe[5] = w1.y, xe[6] = w1.z, xe[7] = w1.w;
+    } else {
+#pragma unroll
+        for (int u = 0; u < VT; ++u) {
+            xb[u] = u < n_mine ? pb[u] : INT32_MAX;
+            xe[u] = u < n_mine ? pe[u] : INT32_MIN;
+        }
+    }
+    // range of every stored value (begins and ends alike, so that any stored
+    // interval lands inside its row's key band)
+    int32_t lo = INT32_MAX, hi = INT32_MIN;
+#pragma unroll
+    for (int u = 0; u < VT; ++u) {
+        const bool ok = u < n_mine;
+        lo = imin(lo, ok ? imin(xb[u], xe[u]) : INT32_MAX);
+        hi = imax(hi, ok ? imax(xb[u], xe[u]) : INT32_MIN);
+    }
+    lo = wave_last(wave_incl_min(lo));
+    hi = wave_last(wave_incl_max(hi));
+    if (lane == 0) {
+        s_red[wave] = lo;
+        s_red[kTW + wave] = hi;
+    }
+    __syncthreads();  // row starts, per-wave ranges
+    {
+        int32_t l = INT32_MAX, h = INT32_MIN;
+        if (lane < kTW) {
+            l = s_red[lane];
+            h = s_red[kTW + lane];
+        }
+        lo = __builtin_amdgcn_readlane(row16_incl_min(l), 15);
+        hi = __builtin_amdgcn_readlane(row16_incl_max(h), 15);
+    }
+    const int64_t range = P > 0 ? (int64_t)hi - (int64_t)lo : 0;
+    const uint32_t shift = range > 0 ? 64u - (uint32_t)__clzll((unsigned long long)range) : 0u;
+    if (((int64_t)nr << shift) >= (1ll << 31)) {  // uniform: rows x key band do not fit 31 bits
+        __syncthreads();  // every thread has read s_red / s_aux before the tile code reuses s_aux
+        flat_tile_whole<OP, TB, VT>(A, B, U, RPT, tile, s_k, *reinterpret_cast<FlatShared<TB>*>(s_aux), lb, epoch,
+                                    out_off, ob, oe, capacity, d_total, status, spin_limit, &s_excl);
+        return;
+    }
+    const int32_t xmin = lo;
+    // keys (tile row << shift) + (x - xmin); a chunk's slots past its
+    // operand's end hold the operand's upper sentinel
+    if (n_mine > 0) {
+        const int32_t* rs = inA ? s_ra : s_rb;
+        int k = 0, kh = nr;  // row of element q0: rs[k] <= q0 < rs[k + 1]
+        while (kh - k > 1) {
+            const int mid = (k + kh) >> 1;
+            if (rs[mid] <= q0) k = mid;
+            else kh = mid;
+        }
+        int32_t nxt = rs[k + 1];
+        uint32_t off = ((uint32_t)k << shift) - (uint32_t)xmin;
+        const int32_t slot = (inA ? a0 : b0) + q0;  // a multiple of 8: the chunk's 8 slots are contiguous
+        int2* dst = s_k + sk(slot);
+#pragma unroll
+        for (int u = 0; u < VT; ++u) {
+            const int32_t q = q0 + u;
+            if (q >= nxt && u < n_mine) {
+                do {
+                    ++k;
+                    nxt = rs[k + 1];
+                } while (q >= nxt);
+                off = ((uint32_t)k << shift) - (uint32_t)xmin;
+            }
+            dst[u] = u < n_mine ? make_int2((int32_t)((uint32_t)xb[u] + off), (int32_t)((uint32_t)xe[u] + off))
+                                : make_int2(INT32_MAX, kHiY);
+        }
+    }
+    if (tid <= VT) {
+        s_k[sk(a0 + nA + tid)] = make_int2(INT32_MAX, kHiY);
+        s_k[sk(b0 + nB + tid)] = make_int2(INT32_MAX, kHiY);
+    }
+    if (tid == 0) {
+        s_k[sk(a0 - 1)] = make_int2(INT32_MIN, INT32_MIN);
+        s_k[sk(b0 - 1)] = make_int2(INT32_MIN, INT32_MIN);
+    }
+    __syncthreads();  // staged keys
+
+    // merge-path walk of positions [p0, p0 + VT)
+    const int32_t p0 = tid * VT;
+    const uint32_t valid = p0 >= P ? 0u : (P - p0 >= VT ? (1u << VT) - 1u : (1u << (P - p0)) - 1u);
+    int32_t rb[VT], re[VT];
+    uint32_t emit = 0;
+    int32_t lm = INT32_MIN;  // union: largest end key before the current position (this thread's part)
+    {
+        int32_t lo2 = p0 - nB > 0 ? p0 - nB : 0, hi2 = p0 < nA ? p0 : nA;
+        if (p0 >= P) lo2 = hi2 = nA;  // positions past the end: every walk reads sentinels
+        while (lo2 < hi2) {
+            const int32_t mid = (lo2 + hi2) >> 1;
+            const int2 ka = s_k[sk(a0 + mid)];
+            const int2 kb = s_k[sk(b0 + (p0 - 1 - mid))];
+            if (key_a_first<OP>(ka.x, kb.x, kb.y)) lo2 = mid + 1;
+            else hi2 = mid;
+        }
+        int32_t ia = a0 + lo2, jb = b0 + (p0 >= P ? nB : p0 - lo2);
+        int32_t cAx, cAy, cBx, cBy, pAx = 0, pAy = 0, pBy = 0;
+        {
+            const int2 t0 = s_k[sk(ia)], t1 = s_k[sk(jb)];
+            cAx = t0.x, cAy = t0.y, cBx = t1.x, cBy = t1.y;
+            if (OP != 0) {
+                const int2 t2 = s_k[sk(ia - 1)], t3 = s_k[sk(jb - 1)];
+                pAx = t2.x, pAy = t2.y, pBy = t3.y;
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < VT; ++v) {
+            const bool isA = key_a_first<OP>(cAx, cBx, cBy);
+            const int32_t nidx = (isA ? ia : jb) + 1;
+            const int2 nraw = s_k[sk(nidx)];
+            if (OP == 0) {
+                const int32_t x = isA ? cAx : cBx;
+                const int32_t y = isA ? cAy : cBy;
+                rb[v] = x;
+                re[v] = lm;  // the largest end key before position v
+                emit |= (x > lm ? 1u : 0u) << v;
+                lm = imax(lm, y);
+            } else if (OP == 1) {
+                const int32_t ceA = pBy > cAx ? pBy : (cBx == cAx ? cBy : INT32_MIN);
+                const bool emA = ceA != INT32_MIN;
+                const bool emB = pAx < cBx && cBx < pAy;
+                emit |= ((isA ? emA : emB) ? 1u : 0u) << v;
+                rb[v] = isA ? cAx : cBx;
+                re[v] = isA ? imin(cAy, ceA) : imin(cBy, pAy);
+            } else {
+                const bool emA = cBx > cAx;
+                const int32_t peA = cBx < cAy ? cBx : cAy;
+                const bool emB = cBy < pAy && nraw.x > cBy;
+                const int32_t peB = nraw.x < pAy ? nraw.x : pAy;
+                emit |= ((isA ? emA : emB) ? 1u : 0u) << v;
+                rb[v] = isA ? cAx : cBy;
+                re[v] = isA ? peA : peB;
+            }
+            if (OP != 0) {
+                pAx = isA ? cAx : pAx;
+                pAy = isA ? cAy : pAy;
+                pBy = isA ? pBy : cBy;
+            }
+            cAx = isA ? nraw.x : cAx;
+            cAy = isA ? nraw.y : cAy;
+            cBx = isA ? cBx : nraw.x;
+            cBy = isA ? cBy : nraw.y;
+            ia = isA ? nidx : ia;
+            jb = isA ? jb : nidx;
+        }
+    }
+    int32_t carry = INT32_MIN;
+    if (OP == 0) {
+        // union: a position starts a run when its begin is above every end
+        // before it; begins grow along the walk, so those above the carry
+        // form a suffix of the thread's positions
+        carry = block_excl_max<kTW>(lm, s_wmax);
+        int32_t below = 0;
+#pragma unroll
+        for (int v = 0; v < VT; ++v) below += rb[v] <= carry ? 1 : 0;
+        emit &= ~((1u << below) - 1u);
+    }
+    emit &= valid;
+    int32_t tile_total = 0;
+    const int32_t my_base = block_excl_sum<kTW>(__popc(emit), s_wsum, &tile_total);
+    if (tid == 0) lookback_publish(lb, tile, (uint32_t)tile_total, epoch);
+    // every walk is done (barrier in the scans): s_k now holds the tile's
+    // outputs, decoded and compacted (pairs [0, tile_total)), and past them
+    // each thread's exclusive count | emit mask << 16 for the row offsets
+    const uint32_t kmask = shift ? (1u << shift) - 1u : 0u;
+    int32_t* s_out = reinterpret_cast<int32_t*>(s_k);
+    uint32_t* s_pk = reinterpret_cast<uint32_t*>(s_k + kPCAP);
+    s_pk[tid] = (uint32_t)my_base | (emit << 16);
+    {
+        int32_t w = my_base;
+#pragma unroll
+        for (int v = 0; v < VT; ++v) {
+            if (emit & (1u << v)) {
+                s_out[2 * w] = (int32_t)(((uint32_t)rb[v] & kmask) + (uint32_t)xmin);
+                if (OP == 0) {
+                    if (w > 0) s_out[2 * w - 1] = (int32_t)(((uint32_t)imax(carry, re[v]) & kmask) + (uint32_t)xmin);
+                } else {
+                    s_out[2 * w + 1] = (int32_t)(((uint32_t)re[v] & kmask) + (uint32_t)xmin);
+                }
+                ++w;
+            }
+        }
+        // union: the owner of the tile's last position closes the last run
+        if (OP == 0 && tile_total > 0 && p0 < P && P - p0 <= VT)
+            s_out[2 * tile_total - 1] = (int32_t)(((uint32_t)imax(carry, lm) & kmask) + (uint32_t)xmin);
+    }
+    if (wave == 0) {
+        const uint32_t e = lookback_exclusive<1, true>(lb, tile, (uint32_t)tile_total, epoch, status, spin_limit);
+        if (lane == 0) s_excl = e;
+    }
+    __syncthreads();
+    const int64_t excl = (int64_t)s_excl;
+    const bool over = excl + tile_total > capacity;
+    if (over && tid == 0 && tile_total > 0) atomicOr(status, STX_FLAG_CAPACITY);
+    if (!over) {
+        int32_t* obt = ob + excl;
+        int32_t* oet = oe + excl;
+        for (int32_t k = tid; k < tile_total; k += TB) {
+            const int2 o = s_k[k];
+            obt[k] = o.x;
+            oet[k] = o.y;
+        }
+    }
+    for (int r = tid; r < nr; r += TB) {  // row r starts at merged position m
+        const int32_t m = s_ra[r] + s_rb[r];
+        const uint32_t pk = s_pk[m >> 3];
+        const uint32_t below = (pk >> 16) & ((1u << (m & 7)) - 1u);
+        out_off[row0 + r] = (int32_t)(excl + (int64_t)(pk & 0xffffu) + __popc(below));
+    }
+    if (row0 + nr == U && tid == 0) {
+        out_off[U] = (int32_t)(excl + tile_total);
+        if (d_total) *d_total = (int32_t)(excl + tile_total);
+    }
+}
+
 // One block: aggregates (flag 1) of tiles [0, ntiles) -> inclusive prefixes
 // (flag 2), same epoch.
 constexpr int kPrefixThreads = 1024;
@@ -1266,6 +1589,21 @@ static void launch_flat(int32_t op, uint32_t ntiles, hipStream_t stream, const O
                                   epoch, spin);
 }
 
+template <int TB, int VT>
+static void launch_flat2(int32_t op, uint32_t ntiles, hipStream_t stream, const Operand& A, const Operand& B, int64_t U,
+                         int32_t rpt, int32_t* out_off, int32_t* ob, int32_t* oe, int64_t capacity, int32_t* d_total,
+                         uint32_t* status, void* scratch, uint32_t epoch, int64_t spin) {
+    if (op == 0)
+        hipLaunchKernelGGL((setop_flat2_kernel<0, TB, VT>), dim3(ntiles), dim3(TB), 0, stream, A, B, U, rpt, ntiles,
+                           out_off, ob, oe, capacity, d_total, status, scratch, epoch, spin);
+    else if (op == 1)
+        hipLaunchKernelGGL((setop_flat2_kernel<1, TB, VT>), dim3(ntiles), dim3(TB), 0, stream, A, B, U, rpt, ntiles,
+                           out_off, ob, oe, capacity, d_total, status, scratch, epoch, spin);
+    else
+        hipLaunchKernelGGL((setop_flat2_kernel<2, TB, VT>), dim3(ntiles), dim3(TB), 0, stream, A, B, U, rpt, ntiles,
+                           out_off, ob, oe, capacity, d_total, status, scratch, epoch, spin);
+}
+
 template <int G>
 static int32_t launch_setop_g(int32_t op, const Operand& A, const Operand& B, const int32_t* d_nrows, int64_t nrows_bound,
                               int32_t* out_off, int32_t* ob, int32_t* oe, int64_t capacity, int32_t* d_total,
@@ -1381,7 +1719,18 @@ int32_t stx_setop(int32_t op, const int32_t* a_beg, const int32_t* a_end, const 
                                 status, scratch, epoch, spin);
             return stx_launch_status("setop_flat_kernel");
         }
-        if (group == -6 || (group == 0 && kFlatDefault)) {
+        const bool plain2 = a_xf_mode == XF_NONE && b_off != nullptr && b_xf_mode == XF_NONE;
+        if ((group == -7 || (group == 0 && kFlatDefault)) && plain2) {
+            // flat tiles in the register-output form, 1024 x 8 (two per CU),
+            // filled to 95 %; other operands take the flat tiles below
+            const int32_t rpt = setop_tile_rows<1024, 8>(elems, nrows_bound, kFlatFill, flat_rows_per_tile<1024>());
+            const uint32_t ntiles = (uint32_t)((nrows_bound + rpt) / rpt);
+            if (lookback_bytes(ntiles) > scratch_bytes) return stx_fail(STX_EINVAL, "setop: scratch too small");
+            launch_flat2<1024, 8>(op, ntiles, stream, A, B, nrows_bound, rpt, out_off, out_beg, out_end, capacity,
+                                  d_total, status, scratch, epoch, spin);
+            return stx_launch_status("setop_flat2_kernel");
+        }
+        if (group == -6 || group == -7 || (group == 0 && kFlatDefault)) {
             // flat tiles, 1024 x 8 (two per CU), filled to 95 %: the mean
             // tile holds ~100 rows or more, so its size varies little
             const int32_t rpt = setop_tile_rows<1024, 8>(elems, nrows_bound, kFlatFill, flat_rows_per_tile<1024>());
@@ -1466,7 +1815,8 @@ int32_t stx_setop_noalloc(int32_t op, const int32_t* a_beg, const int32_t* a_end
     const Operand A{a_beg, a_end, a_off, nullptr, XForm{XF_NONE, 0}};
     const Operand B{b_beg, b_end, b_off, nullptr, XForm{XF_NONE, 0}};
     const int64_t elems = a_n + (b_off ? b_n : 0);
-    // the flat 1024 x 8 tiles of stx_setop's default path
+    // the flat 1024 x 8 tiles of stx_setop's default path (register-output
+    // form for two operands)
     const int32_t rpt = setop_tile_rows<1024, 8>(elems, nrows, kFlatFill, flat_rows_per_tile<1024>());
     const int64_t ntiles = (nrows + rpt) / rpt;
     if (lookback_bytes(ntiles) > kNoAllocScratchBytes)
@@ -1477,6 +1827,11 @@ int32_t stx_setop_noalloc(int32_t op, const int32_t* a_beg, const int32_t* a_end
     uint32_t epoch = 0;
     const int32_t rc = noalloc_acquire(stream, &sp, &epoch);
     if (rc) return rc;
+    if (b_off != nullptr) {
+        launch_flat2<1024, 8>(op, (uint32_t)ntiles, stream, A, B, nrows, rpt, out_off, out_beg, out_end, capacity,
+                              d_total, status, sp, epoch, stx_spin_limit());
+        return stx_launch_status("setop_flat2_kernel(noalloc)");
+    }
     launch_flat<1024, 8>(op, (uint32_t)ntiles, stream, A, B, nrows, rpt, out_off, out_beg, out_end, capacity, d_total,
                          status, sp, epoch, stx_spin_limit());
     return stx_launch_status("setop_flat_kernel(noalloc)");

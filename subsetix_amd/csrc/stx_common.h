@@ -275,22 +275,48 @@ __device__ __forceinline__ int32_t wave_incl_min(int32_t x) {
 // Value of lane 63 (the wave total after an inclusive scan).
 __device__ __forceinline__ int32_t wave_last(int32_t x) { return __builtin_amdgcn_readlane(x, 63); }
 
+// Inclusive scans inside each 16-lane row (row_shr 1, 2, 4, 8).
+__device__ __forceinline__ int32_t row16_incl_sum(int32_t x) {
+    x += dpp_mov<0x111>(0, x);
+    x += dpp_mov<0x112>(0, x);
+    x += dpp_mov<0x114>(0, x);
+    x += dpp_mov<0x118>(0, x);
+    return x;
+}
+
+__device__ __forceinline__ int32_t row16_incl_max(int32_t x) {
+    x = imax(x, dpp_mov<0x111>(INT32_MIN, x));
+    x = imax(x, dpp_mov<0x112>(INT32_MIN, x));
+    x = imax(x, dpp_mov<0x114>(INT32_MIN, x));
+    x = imax(x, dpp_mov<0x118>(INT32_MIN, x));
+    return x;
+}
+
+__device__ __forceinline__ int32_t row16_incl_min(int32_t x) {
+    x = imin(x, dpp_mov<0x111>(INT32_MAX, x));
+    x = imin(x, dpp_mov<0x112>(INT32_MAX, x));
+    x = imin(x, dpp_mov<0x114>(INT32_MAX, x));
+    x = imin(x, dpp_mov<0x118>(INT32_MAX, x));
+    return x;
+}
+
 // Block-wide exclusive sum over one value per thread with one barrier:
-// `lds` holds one slot per wave and is used once per kernel.
+// `lds` holds one slot per wave and is used once per kernel.  After the
+// barrier lane w of every wave holds wave w's total, and one row or wave
+// scan of those gives the prefix and the block total.
 template <int NW>
 __device__ __forceinline__ int32_t block_excl_sum(int32_t v, int32_t* lds, int32_t* total) {
-    const int wave = wave_id();
+    static_assert(NW >= 1 && NW <= 64, "at most 64 waves");
+    const int wave = wave_id(), lane = lane_id();
     const int32_t incl = wave_incl_sum(v);
-    if (lane_id() == 63) lds[wave] = incl;
+    if (lane == 63) lds[wave] = incl;
     __syncthreads();
-    int32_t before = 0, all = 0;
-#pragma unroll
-    for (int w = 0; w < NW; ++w) {
-        const int32_t t = lds[w];
-        before += w < wave ? t : 0;
-        all += t;
-    }
-    *total = all;
+    int32_t t = 0;
+    if (lane < NW) t = lds[lane];
+    t = NW <= 16 ? row16_incl_sum(t) : wave_incl_sum(t);
+    const int wv = __builtin_amdgcn_readfirstlane(wave);
+    const int32_t before = wv > 0 ? __builtin_amdgcn_readlane(t, wv > 0 ? wv - 1 : 0) : 0;
+    *total = __builtin_amdgcn_readlane(t, NW - 1);
     return before + incl - v;
 }
 
@@ -298,13 +324,16 @@ __device__ __forceinline__ int32_t block_excl_sum(int32_t v, int32_t* lds, int32
 // holds one slot per wave and is used once per kernel.
 template <int NW>
 __device__ __forceinline__ int32_t block_excl_max(int32_t v, int32_t* lds) {
-    const int wave = wave_id();
+    static_assert(NW >= 1 && NW <= 64, "at most 64 waves");
+    const int wave = wave_id(), lane = lane_id();
     const int32_t incl = wave_incl_max(v);
-    if (lane_id() == 63) lds[wave] = incl;
+    if (lane == 63) lds[wave] = incl;
     __syncthreads();
-    int32_t before = INT32_MIN;
-#pragma unroll
-    for (int w = 0; w < NW; ++w) before = w < wave ? imax(before, lds[w]) : before;
+    const int wv = __builtin_amdgcn_readfirstlane(wave);
+    int32_t t = INT32_MIN;
+    if (lane < wv) t = lds[lane];  // the waves before this one
+    t = NW <= 16 ? row16_incl_max(t) : wave_incl_max(t);
+    const int32_t before = __builtin_amdgcn_readlane(t, NW <= 16 ? 15 : 63);
     // wave_shr:1 - the inclusive value of the lane below (lane 0: identity)
     return imax(before, dpp_mov<0x138>(INT32_MIN, incl));
 }

@@ -55,7 +55,7 @@ def test_golden_cell_model(sx, golden):
             assert ox.cells(to_host(got)) == cell_model(spec, cells), spec
 
 
-@pytest.mark.parametrize("group", [None, -2, -3, -4, -5, -6, 4, 16, 64])  # None / negative: LDS tile shapes
+@pytest.mark.parametrize("group", [None, -2, -3, -4, -5, -6, -7, 4, 16, 64])  # None / negative: LDS tile shapes
 @pytest.mark.parametrize("seed", [0, 1, 2])
 def test_random_vs_oracle(sx, seed, group):
     from subsetix_amd import _core
@@ -197,14 +197,15 @@ def test_row_count_multiple_of_tile_rows(sx):
             assert_set_equal(to_host(_ops(sx)[op](da, db)), ox.setop(op, a, b))
 
 
-@pytest.mark.parametrize("schedule", [-1, -2, -3, -4, -5, -6])
+@pytest.mark.parametrize("schedule", [-1, -2, -3, -4, -5, -6, -7])
 def test_tile_schedules_batched_pairs(sx, schedule):
     """Tile schedules on 16 stacked full-size pairs (hundreds of tiles),
     bit-exact against the oracle: single pass 512 x 8 (-1, decoupled
     look-back), two passes (-2, count kernel, prefix kernel, write kernel),
     single pass 128 x 16 (-3) and 64 x 32 (-4), flat tiles 512 x 8 (-5) and
-    1024 x 8 (-6); repeated launches on one scratch buffer alternate schedules.  Dilation (transforms on load, one
-    operand) runs through the same shape."""
+    1024 x 8 (-6), flat tiles in the register-output form (-7); repeated launches on one scratch buffer alternate
+    schedules.  Dilation (transforms on load, one operand) runs through the same shape (-7 takes the -6 tiles
+    there)."""
     import torch
 
     from subsetix_amd import _core, workloads
@@ -295,7 +296,7 @@ def _flat_edge_pair(seed):
     return _rows_to_oset(A), _rows_to_oset(B)
 
 
-@pytest.mark.parametrize("schedule", [-1, -5, -6])
+@pytest.mark.parametrize("schedule", [-1, -5, -6, -7])
 @pytest.mark.parametrize("seed", [0, 1])
 def test_flat_tile_edge_cases(sx, schedule, seed):
     """Every branch of the flat tile kernel (and the same inputs through the
