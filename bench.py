@@ -12,7 +12,8 @@ Also reported beside the headline (each in its own object):
   copy           1 GiB device copy: the achievable-bandwidth reference and the
                  calibration kernel for the HBM byte counters
   plain_form     one pair through the reference-shaped evaluate() call
-  morphology     dilate / erode, square SE radius 1, 2, 4 (one pair)
+  morphology     dilate / erode, square SE radius 1, 2, 4 (one pair), and dilation of
+                 64 instances stacked with 16-row gaps in one call
   large_stencil  8192^2 fine / 4096^2 coarse composite, 500 steps, FP32 vs FP64
   batched_amr    512 three-level instances, 200 steps, regrid every 8, device
                  regrid, each regrid + 8 steps one hipGraph replay; exclusive
@@ -342,6 +343,7 @@ def main():
             extra[s] = perf.copy_bandwidth(dev)
         elif s == "morphology":
             extra[s] = perf.morphology(pair, 10, dev)
+            extra[s]["batched"] = perf.morphology_batched(64, 5, dev)
         elif s == "large_stencil":
             extra[s] = perf.large_stencil(args.stencil_steps, dev)
         elif s == "batched_amr":

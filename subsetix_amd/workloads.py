@@ -77,6 +77,24 @@ def stack_rows(sets) -> HostSet:
     return HostSet(off, np.concatenate(begs), np.concatenate(ends), np.arange(nrows, dtype=np.int32))
 
 
+def stack_rows_gapped(sets, gap: int) -> HostSet:
+    """Concatenate independent sets row-wise keeping their row ids, each
+    set's rows shifted past the previous set's last row plus ``gap`` empty
+    rows: the batched form of a row-window operation (dilation by up to
+    ``gap`` / 2 rows keeps the instances apart)."""
+    offs, begs, ends, rows = [], [], [], []
+    base, shift = 0, 0
+    for s in sets:
+        offs.append(s.row_offsets[:-1].astype(np.int64) + base)
+        begs.append(s.begin)
+        ends.append(s.end)
+        rows.append(s.rows.astype(np.int64) + shift)
+        base += s.n
+        shift += (int(s.rows[-1]) + 1 if s.rows.size else 0) + gap
+    off = np.concatenate(offs + [np.array([base])]).astype(np.int32)
+    return HostSet(off, np.concatenate(begs), np.concatenate(ends), np.concatenate(rows).astype(np.int32))
+
+
 # ---------------------------------------------------------------------------
 # Shapes -> interval sets (host, vectorised)
 

@@ -79,6 +79,33 @@ def test_setalgebra_config_morphology_full_size(sx, r):
                      ox.erode(a, r, r, width=65536, height=4096))
 
 
+@pytest.mark.parametrize("r", [1, 4])
+def test_dilate_stacked_instances_vs_oracle(sx, r):
+    """The batched form of the morphology benchmark: set-algebra instances
+    stacked with 16 empty rows between them, dilated in one call, equal the
+    oracle's dilation of each instance with its rows shifted (instances of
+    4096 rows and a few short ones, so row windows cross every gap)."""
+    from subsetix_amd import morphology as m
+    from subsetix_amd import workloads
+
+    sets = [workloads.setalgebra_pair(seed_a=1 + 2 * k, seed_b=2 + 2 * k)[0] for k in range(3)]
+    sets += [workloads.random_rows_set(n, 65536, seed=40 + n) for n in (5, 1, 37)]
+    stacked = workloads.stack_rows_gapped(sets, 16)
+    print("stacked dilate r", r, "rows", stacked.rows.size, flush=True)
+    got = to_host(m.dilate_interval_set(to_device(hostset_to_oset(stacked)), r, r))
+    rows, off, beg, end = [], [0], [], []
+    shift = 0
+    for h in sets:
+        d = ox.dilate(hostset_to_oset(h), r, r)
+        rows.append(d.rows.astype(np.int64) + shift)
+        off.extend((d.off[1:].astype(np.int64) + off[-1]).tolist())
+        beg.append(d.beg)
+        end.append(d.end)
+        shift += int(h.rows[-1]) + 1 + 16
+    exp = ox.make(np.array(off), np.concatenate(beg), np.concatenate(end), rows=np.concatenate(rows))
+    assert_set_equal(got, exp)
+
+
 def test_translate_and_boundaries(sx):
     from subsetix_amd import morphology as m
 
