@@ -126,6 +126,13 @@ int32_t stx_interval_rows(const int32_t* off, int64_t R, const int32_t* rows, in
 /* Dense row map for clip_interval_set (morphology.py:268-319): map[y] =
  * index of row y0+y in rows, or -1. */
 int32_t stx_rows_lookup(const int32_t* rows, int64_t R, int32_t y0, int64_t H, int32_t* map, hipStream_t stream);
+/* Dense offsets of a set over the rows y0 .. y0 + n - 1 (n + 1 entries):
+ * out[j] = intervals in rows below y0 + j, counted from off[0]; rows the set
+ * does not hold are empty.  Row-shifted operands of the vertical dilation
+ * (morphology.py:211-248 unions 2 hy + 1 row translates) are views of such
+ * offsets. */
+int32_t stx_dense_offsets(const int32_t* rows, const int32_t* off, int64_t R, int64_t y0, int64_t n, int32_t* out,
+                          hipStream_t stream);
 /* Drop rows without intervals (the unique(rows) of the rebuild,
  * morphology.py:43-48).  tmp: 2*(R+1) int32. */
 int32_t stx_compact_rows(const int32_t* rows, const int32_t* off, int64_t R, int32_t shift, int32_t* rows_out,
@@ -164,6 +171,20 @@ int32_t stx_fill_i32(int32_t* p, int64_t n, int32_t value, int32_t step, hipStre
  * intervals.  tmp: stx_dilate_tmp_ints(R, hy) int32.  Three scans on the
  * scratch use epochs epoch, epoch + 1, epoch + 2. */
 int32_t stx_dilate_max_halo(void);
+/* The same dilation over the dense output rows row_lo - hy .. row_hi + hy
+ * (row_lo / row_hi: the set's first and last row id; nt = row_hi - row_lo +
+ * 1 + 2 hy): tiles of consecutive output rows each merge their 2 hy + 1
+ * widened input rows in LDS, then the rows without intervals are dropped.
+ * Outputs as stx_dilate (row_cap >= nt, capacity >= (2 hy + 1) N).  tmp:
+ * stx_dilate_dense_tmp_ints(nt, hy) int32; scratch:
+ * stx_dilate_dense_scratch_bytes(nt) with epochs epoch and epoch + 1. */
+int64_t stx_dilate_dense_tmp_ints(int64_t nt, int32_t hy);
+int64_t stx_dilate_dense_scratch_bytes(int64_t nt);
+int32_t stx_dilate_dense(const int32_t* rows, const int32_t* off, int64_t R, const int32_t* beg, const int32_t* end,
+                         int64_t N, int32_t hx, int32_t hy, int32_t row_lo, int32_t row_hi, int32_t* rows_out,
+                         int32_t* off_out, int32_t* beg_out, int32_t* end_out, int64_t row_cap, int64_t capacity,
+                         int32_t* d_ny, int32_t* d_total, int32_t* tmp, int64_t tmp_ints, uint32_t* status,
+                         void* scratch, int64_t scratch_bytes, uint32_t epoch, hipStream_t stream);
 int64_t stx_dilate_tmp_ints(int64_t R, int32_t hy);
 int32_t stx_dilate(const int32_t* rows, const int32_t* off, int64_t R, const int32_t* beg, const int32_t* end,
                    int32_t hx, int32_t hy, int32_t* rows_out, int32_t* off_out, int32_t* beg_out, int32_t* end_out,

@@ -20,6 +20,7 @@ I32 = torch.int32
 _lib.register("stx_validate_csr", _I32, [_P, _I64, _P, _I64, _P, _P])
 _lib.register("stx_interval_rows", _I32, [_P, _I64, _P, _I64, _P, _P, _P])
 _lib.register("stx_rows_lookup", _I32, [_P, _I64, _I32, _I64, _P, _P])
+_lib.register("stx_dense_offsets", _I32, [_P, _P, _I64, _I64, _I64, _P, _P])
 _lib.register("stx_compact_rows", _I32, [_P, _P, _I64, _I32, _P, _P, _P, _P, _P, _P, _I64, _I32, _P])
 _lib.register("stx_coarse_rows", _I32, [_P, _P, _I64, _I32, _I32, _P, _P, _I64, _P, _P, _P, _P, _I64, _I32, _P])
 _lib.register("stx_prolong_set", _I32, [_P, _P, _I64, _P, _P, _I64, _I32, _P, _P, _P, _P, _P, _P])

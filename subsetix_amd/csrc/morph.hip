@@ -149,7 +149,331 @@ void launch_union(bool write, const int32_t* off_c, const int32_t* beg, const in
                            d_ny, NYb, klo, khi, hx, cnt, out_off, ob, oe, capacity, status);
 }
 
+// ---------------------------------------------------------------------------
+// Tiled dilation over dense rows.
+//
+// The rows lo - hy .. hi + hy of the output are numbered i = 0 .. nt - 1
+// (row lo - hy + i) and the input is read through dense offsets D over the
+// rows lo - 2 hy + j (j = 0 .. nt + 2 hy), so output row i is the union of
+// the input rows j = i .. i + 2 hy, each interval widened by hx.  A tile
+// takes nr consecutive output rows [i0, i0 + nr): it stages the input rows
+// [i0, i0 + nr + 2 hy) in LDS as one keyed sequence (key = (local row <<
+// shift) + (x - xmin), begins sorted since rows are), and forms the window
+// union by doubling on that sequence: W1 = the staged rows, W2q(l) = Wq(l) U
+// Wq(l + q) for q = 1, 2, .. < p (p the largest power of two <= 2 hy + 1),
+// then out(l) = Wp(l) U Wp(l + 2 hy + 1 - p) for l < nr.  Every step is one
+// merge-path union of the sequence with its own suffix from row q, the
+// suffix's keys lowered by q rows (begins stay sorted; a run never crosses a
+// row since 2^shift exceeds every key offset), compacted into the other of
+// two LDS buffers.  The tile's count then goes through the look-back, and
+// one coalesced copy decodes the keys into the output.  A tile whose input
+// or one of whose steps exceeds an LDS buffer, or whose keys do not fit 30
+// bits, merges each of its output rows in one thread instead (the 2 hy + 1
+// input rows as cursors, a count pass and a write pass).
+constexpr int kDTB = 1024;    // threads per tile
+constexpr int kDVT = 8;       // merge positions per thread and step
+constexpr int kDCap = 4096;   // intervals of one LDS buffer (one step merges <= 2 kDCap = kDTB * kDVT)
+constexpr int kDRows = 1040;  // staged row starts (nr + 2 hy + 1 <= 1024 + 8 + 1)
+static_assert(2 * kDCap == kDTB * kDVT, "a step merges a buffer with its suffix");
+
+__device__ __forceinline__ int32_t dsk(int32_t i) { return i + (i >> 4); }  // one spare slot per 16: fewer bank conflicts
+constexpr int dsk_size(int n) { return n + n / 16 + 1; }
+
+// First index in [0, n) whose key is >= key (n if none).
+__device__ __forceinline__ int32_t lds_lower_bound(const int2* X, int32_t n, int32_t key) {
+    int32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (X[dsk(mid)].x < key) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Union of the dense input rows j = i .. i + K - 1, widened by hx: the
+// interval count (WRITE = false) or the intervals written from ob/oe + w.
+template <int KMAX, bool WRITE>
+__device__ int32_t dense_window_union(const int32_t* __restrict__ D, const int32_t* __restrict__ beg,
+                                      const int32_t* __restrict__ end, int64_t i, int K, int32_t hx,
+                                      int32_t* __restrict__ ob, int32_t* __restrict__ oe, int64_t w, int64_t capacity,
+                                      uint32_t* __restrict__ status) {
+    int32_t j[KMAX], je[KMAX], hb[KMAX];  // cursor, end, head begin (INT_MAX when done)
+#pragma unroll
+    for (int t = 0; t < KMAX; ++t) {
+        j[t] = t < K ? D[i + t] : 0;
+        je[t] = t < K ? D[i + t + 1] : 0;
+        hb[t] = j[t] < je[t] ? beg[j[t]] : INT_MAX;
+    }
+    int32_t n = 0, s = 0, m = INT_MIN;
+    bool open = false;
+    while (true) {
+        int32_t best = INT_MAX, tb = -1;
+#pragma unroll
+        for (int t = 0; t < KMAX; ++t)
+            if (hb[t] < best) { best = hb[t]; tb = t; }
+        if (tb < 0) break;
+        int32_t e = 0;
+#pragma unroll
+        for (int t = 0; t < KMAX; ++t) {
+            if (t == tb) {
+                e = end[j[t]];
+                ++j[t];
+                hb[t] = j[t] < je[t] ? beg[j[t]] : INT_MAX;
+            }
+        }
+        const int32_t b = best - hx;
+        e += hx;
+        if (!open || b > m) {
+            if (open) {
+                if (WRITE) {
+                    if (w < capacity) { ob[w] = s; oe[w] = m; } else atomicOr(status, STX_FLAG_CAPACITY);
+                    ++w;
+                }
+                ++n;
+            }
+            s = b;
+            m = e;
+            open = true;
+        } else if (e > m) {
+            m = e;
+        }
+    }
+    if (open) {
+        if (WRITE) {
+            if (w < capacity) { ob[w] = s; oe[w] = m; } else atomicOr(status, STX_FLAG_CAPACITY);
+        }
+        ++n;
+    }
+    return n;
+}
+
+template <int KMAX>
+__global__ __launch_bounds__(kDTB) __attribute__((amdgpu_waves_per_eu(8))) void dilate_tile_kernel(
+    const int32_t* __restrict__ D, const int32_t* __restrict__ beg, const int32_t* __restrict__ end, int64_t nt,
+    int32_t hx, int32_t hy, int32_t rpt, uint32_t ntiles, int32_t* __restrict__ out_off, int32_t* __restrict__ ob,
+    int32_t* __restrict__ oe, int64_t capacity, int32_t* __restrict__ d_total, uint32_t* status, void* scratch,
+    uint32_t epoch, int64_t spin_limit) {
+    constexpr int kTW = kDTB / STX_WAVE;
+    constexpr int kPerThread = kDCap / kDTB;
+    __shared__ __align__(16) int2 s_buf[2][dsk_size(kDCap)];
+    __shared__ int32_t s_rs[kDRows];
+    __shared__ int32_t s_red[2 * kTW];
+    __shared__ int32_t s_wmax[kTW];
+    __shared__ int32_t s_wsum[kTW];
+    __shared__ uint32_t s_excl;
+    LookbackScratch lb = lookback_scratch(scratch, ntiles);
+    const uint32_t tile = blockIdx.x;
+    const int tid = threadIdx.x, lane = lane_id(), wave = wave_id();
+    const int K = 2 * hy + 1;
+    const int64_t i0 = (int64_t)tile * rpt;
+    const int nr = (int)(nt - i0 < rpt ? nt - i0 : rpt);  // ntiles = ceil(nt / rpt): nr >= 1
+    const int Rt = nr + 2 * hy;                            // staged input rows [i0, i0 + Rt)
+    const int32_t base = D[i0];
+    const int32_t nIn = D[i0 + Rt] - base;
+    bool lds_ok = nIn <= kDCap;  // uniform
+    for (int r = tid; r <= Rt; r += kDTB) s_rs[r] = D[i0 + r] - base;
+    int32_t xb[kPerThread], xe[kPerThread];
+    int32_t lo = INT32_MAX, hi = INT32_MIN;
+    if (lds_ok) {
+#pragma unroll
+        for (int u = 0; u < kPerThread; ++u) {
+            const int32_t q = tid + u * kDTB;
+            xb[u] = q < nIn ? beg[base + q] : INT32_MAX;
+            xe[u] = q < nIn ? end[base + q] : INT32_MIN;
+            lo = imin(lo, xb[u]);
+            hi = imax(hi, xe[u]);
+        }
+    }
+    lo = wave_last(wave_incl_min(lo));
+    hi = wave_last(wave_incl_max(hi));
+    if (lane == 0) {
+        s_red[wave] = lo;
+        s_red[kTW + wave] = hi;
+    }
+    __syncthreads();  // row starts, per-wave ranges
+    {
+        int32_t l = INT32_MAX, h = INT32_MIN;
+        if (lane < kTW) {
+            l = s_red[lane];
+            h = s_red[kTW + lane];
+        }
+        lo = __builtin_amdgcn_readlane(row16_incl_min(l), 15);
+        hi = __builtin_amdgcn_readlane(row16_incl_max(h), 15);
+    }
+    // widened values span [lo - hx, hi + hx]; keys below 2^30 leave room for
+    // the row shifts
+    const int64_t range = nIn > 0 ? (int64_t)hi - (int64_t)lo + 2 * (int64_t)hx : 0;
+    const uint32_t shift = range > 0 ? 64u - (uint32_t)__clzll((unsigned long long)range) : 0u;
+    lds_ok = lds_ok && range >= 0 && shift < 31 && ((int64_t)Rt << shift) < (1ll << 30);
+    const int32_t xmin = (int32_t)((uint32_t)lo - (uint32_t)hx);
+    int32_t total = 0;
+    int cur = 0;
+    if (lds_ok) {
+        // stage: interval q of local row k -> key (k << shift) + (x - xmin)
+#pragma unroll
+        for (int u = 0; u < kPerThread; ++u) {
+            const int32_t q = tid + u * kDTB;
+            if (q < nIn) {
+                int k = 0, kh = Rt;  // s_rs[k] <= q < s_rs[k + 1]
+                while (kh - k > 1) {
+                    const int mid = (k + kh) >> 1;
+                    if (s_rs[mid] <= q) k = mid;
+                    else kh = mid;
+                }
+                const uint32_t off = ((uint32_t)k << shift) - (uint32_t)xmin;
+                s_buf[0][dsk(q)] = make_int2((int32_t)((uint32_t)xb[u] - (uint32_t)hx + off),
+                                             (int32_t)((uint32_t)xe[u] + (uint32_t)hx + off));
+            }
+        }
+        __syncthreads();  // staged keys
+        int p = 1;
+        while (2 * p <= K) p *= 2;
+        int32_t nX = nIn;
+        for (int q = 1;; q *= 2) {
+            const bool last = q >= p;
+            const int dq = last ? K - p : q;  // rows the suffix is lowered by
+            const int2* X = s_buf[cur];
+            int2* Y = s_buf[cur ^ 1];
+            const int32_t delta = (int32_t)((uint32_t)dq << shift);
+            const int32_t nA = last ? lds_lower_bound(X, nX, (int32_t)((uint32_t)nr << shift)) : nX;
+            const int32_t b0 = lds_lower_bound(X, nX, delta);
+            const int32_t b1 = last ? lds_lower_bound(X, nX, (int32_t)((uint32_t)(nr + dq) << shift)) : nX;
+            const int32_t nB = b1 - b0;
+            const int32_t P = nA + nB;
+            const int32_t p0 = tid * kDVT;
+            const uint32_t valid =
+                p0 >= P ? 0u : (P - p0 >= kDVT ? (1u << kDVT) - 1u : (1u << (P - p0)) - 1u);
+            int32_t rb[kDVT], re[kDVT];
+            uint32_t emit = 0;
+            int32_t lm = INT32_MIN;  // largest end key before the current position (this thread's part)
+            {
+                int32_t ia = nA, jb = nB;  // positions past the end read sentinels only
+                if (p0 < P) {
+                    int32_t l2 = p0 - nB > 0 ? p0 - nB : 0, h2 = p0 < nA ? p0 : nA;
+                    while (l2 < h2) {
+                        const int32_t mid = (l2 + h2) >> 1;
+                        if (X[dsk(mid)].x <= X[dsk(b0 + p0 - 1 - mid)].x - delta) l2 = mid + 1;
+                        else h2 = mid;
+                    }
+                    ia = l2;
+                    jb = p0 - l2;
+                }
+                int2 ca = ia < nA ? X[dsk(ia)] : make_int2(INT32_MAX, INT32_MIN);
+                int2 cb = make_int2(INT32_MAX, INT32_MIN);
+                if (jb < nB) {
+                    cb = X[dsk(b0 + jb)];
+                    cb.x -= delta;
+                    cb.y -= delta;
+                }
+#pragma unroll
+                for (int v = 0; v < kDVT; ++v) {
+                    const bool isA = ca.x <= cb.x;
+                    const int32_t x = isA ? ca.x : cb.x;
+                    const int32_t y = isA ? ca.y : cb.y;
+                    rb[v] = x;
+                    re[v] = lm;
+                    emit |= (x > lm ? 1u : 0u) << v;
+                    lm = imax(lm, y);
+                    if (isA) {
+                        ++ia;
+                        ca = ia < nA ? X[dsk(ia)] : make_int2(INT32_MAX, INT32_MIN);
+                    } else {
+                        ++jb;
+                        if (jb < nB) {
+                            cb = X[dsk(b0 + jb)];
+                            cb.x -= delta;
+                            cb.y -= delta;
+                        } else {
+                            cb = make_int2(INT32_MAX, INT32_MIN);
+                        }
+                    }
+                }
+            }
+            // a position starts a run when its begin is above every end
+            // before it; begins grow along the walk, so those at or below
+            // the carry form a prefix of the thread's positions
+            const int32_t carry = block_excl_max<kTW>(lm, s_wmax);
+            int32_t below = 0;
+#pragma unroll
+            for (int v = 0; v < kDVT; ++v) below += rb[v] <= carry ? 1 : 0;
+            emit &= ~((1u << below) - 1u);
+            emit &= valid;
+            int32_t step_total = 0;
+            const int32_t my_base = block_excl_sum<kTW>(__popc(emit), s_wsum, &step_total);
+            if (step_total > kDCap) {  // uniform
+                lds_ok = false;
+                break;
+            }
+            int32_t w = my_base;
+#pragma unroll
+            for (int v = 0; v < kDVT; ++v) {
+                if (emit & (1u << v)) {
+                    Y[dsk(w)].x = rb[v];
+                    if (w > 0) Y[dsk(w - 1)].y = imax(carry, re[v]);
+                    ++w;
+                }
+            }
+            // the owner of the last position closes the last run
+            if (step_total > 0 && p0 < P && P - p0 <= kDVT) Y[dsk(step_total - 1)].y = imax(carry, lm);
+            __syncthreads();  // Y complete; every read of X done
+            cur ^= 1;
+            nX = step_total;
+            if (last) break;
+        }
+        total = nX;
+    }
+    int32_t ex = 0;  // row-merge path: this thread's row's exclusive output count
+    if (!lds_ok) {
+        __syncthreads();  // s_red is read by the range combine above
+        const int32_t c = tid < nr ? dense_window_union<KMAX, false>(D, beg, end, i0 + tid, K, hx, nullptr, nullptr, 0,
+                                                                     0, status)
+                                   : 0;
+        ex = block_excl_sum<kTW>(c, s_red, &total);
+    }
+    if (wave == 0) {
+        const uint32_t e = lookback_exclusive<1, false>(lb, tile, (uint32_t)total, epoch, status, spin_limit);
+        if (lane == 0) s_excl = e;
+    }
+    __syncthreads();
+    const int64_t excl = (int64_t)s_excl;
+    const bool over = excl + total > capacity;
+    if (over && tid == 0 && total > 0) atomicOr(status, STX_FLAG_CAPACITY);
+    if (lds_ok) {
+        const int2* X = s_buf[cur];
+        for (int32_t k = tid; k < total; k += kDTB) {
+            const int2 o = X[dsk(k)];
+            const int32_t r = o.x >> shift;
+            if (!over) {
+                const uint32_t boff = ((uint32_t)r << shift) - (uint32_t)xmin;
+                ob[excl + k] = (int32_t)((uint32_t)o.x - boff);
+                oe[excl + k] = (int32_t)((uint32_t)o.y - boff);
+            }
+            // rows (row of output k - 1, row of output k] start at output k
+            const int32_t rp = k > 0 ? (X[dsk(k - 1)].x >> shift) : -1;
+            for (int32_t rr = rp + 1; rr <= r; ++rr) out_off[i0 + rr] = (int32_t)(excl + k);
+        }
+        const int32_t rl = total > 0 ? (X[dsk(total - 1)].x >> shift) : -1;
+        for (int32_t rr = rl + 1 + tid; rr < nr; rr += kDTB) out_off[i0 + rr] = (int32_t)(excl + total);
+    } else if (tid < nr) {
+        out_off[i0 + tid] = (int32_t)(excl + ex);
+        if (!over) dense_window_union<KMAX, true>(D, beg, end, i0 + tid, K, hx, ob, oe, excl + ex, capacity, status);
+    }
+    if (i0 + nr == nt && tid == 0) {
+        out_off[nt] = (int32_t)(excl + total);
+        if (d_total) *d_total = (int32_t)(excl + total);
+    }
+}
+
+__global__ void dense_rows_kernel(int32_t* __restrict__ rows, int64_t n, int32_t y0) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) rows[i] = (int32_t)((int64_t)y0 + i);
+}
+
 }  // namespace
+
+extern "C" int64_t stx_scan_scratch_bytes(int64_t n_bound);
+extern "C" int32_t stx_dense_offsets(const int32_t* rows, const int32_t* off, int64_t R, int64_t y0, int64_t n,
+                                     int32_t* out, hipStream_t stream);
 
 extern "C" {
 
@@ -202,6 +526,64 @@ int32_t stx_dilate(const int32_t* rows, const int32_t* off, int64_t R, const int
         else launch_union<9>(true, off_c, beg, end, d_ny, NYb, klo, khi, hx, nullptr, off_out, beg_out, end_out, capacity, status, stream);
     }
     return stx_launch_status("dilate");
+}
+
+int64_t stx_dilate_dense_tmp_ints(int64_t nt, int32_t hy) {
+    // dense input offsets, dense output offsets, dense row ids, compaction tmp
+    return (nt + 2 * (int64_t)hy + 1) + (nt + 1) + nt + 2 * (nt + 1);
+}
+
+int64_t stx_dilate_dense_scratch_bytes(int64_t nt) {
+    const int64_t lb = lookback_bytes(nt), sc = stx_scan_scratch_bytes(nt);
+    return lb > sc ? lb : sc;
+}
+
+int32_t stx_dilate_dense(const int32_t* rows, const int32_t* off, int64_t R, const int32_t* beg, const int32_t* end,
+                         int64_t N, int32_t hx, int32_t hy, int32_t row_lo, int32_t row_hi, int32_t* rows_out,
+                         int32_t* off_out, int32_t* beg_out, int32_t* end_out, int64_t row_cap, int64_t capacity,
+                         int32_t* d_ny, int32_t* d_total, int32_t* tmp, int64_t tmp_ints, uint32_t* status,
+                         void* scratch, int64_t scratch_bytes, uint32_t epoch, hipStream_t stream) {
+    if (R < 1 || N < 0 || hx < 0 || hy < 1 || hy > kMaxHalo || row_hi < row_lo)
+        return stx_fail(STX_EINVAL, "dilate_dense: need R >= 1, N >= 0, hx >= 0, 1 <= hy <= 4, row_lo <= row_hi");
+    if ((int64_t)row_lo - hy < INT32_MIN || (int64_t)row_hi + hy > INT32_MAX)
+        return stx_fail(STX_EINVAL, "dilate_dense: dilated row ids leave the int32 range");
+    const int64_t span = (int64_t)row_hi - row_lo + 1;
+    const int64_t nt = span + 2 * (int64_t)hy;
+    if (row_cap < nt) return stx_fail(STX_EINVAL, "dilate_dense: row capacity below the dense row count");
+    if (tmp_ints < stx_dilate_dense_tmp_ints(nt, hy)) return stx_fail(STX_EINVAL, "dilate_dense: tmp too small");
+    if (scratch_bytes < stx_dilate_dense_scratch_bytes(nt)) return stx_fail(STX_EINVAL, "dilate_dense: scratch too small");
+    int32_t* D = tmp;
+    int32_t* off_d = D + (nt + 2 * (int64_t)hy + 1);
+    int32_t* rows_d = off_d + (nt + 1);
+    int32_t* ctmp = rows_d + nt;
+    int32_t rc = stx_dense_offsets(rows, off, R, (int64_t)row_lo - 2 * hy, nt + 2 * hy, D, stream);
+    if (rc) return rc;
+    // staged input per tile ~ kDFill of a buffer at the mean row length, so
+    // that the steps (which may grow the count) rarely overflow
+    constexpr double kDFill = 0.7;
+    const double mean = (double)N / (double)span;
+    int64_t rpt = (int64_t)(kDFill * kDCap / (mean > 1.0 ? mean : 1.0)) - 2 * hy;
+    if (rpt < 1) rpt = 1;
+    if (rpt > kDTB) rpt = kDTB;
+    const int64_t ntiles = (nt + rpt - 1) / rpt;
+    const int K = 2 * hy + 1;
+    if (K <= 3)
+        hipLaunchKernelGGL((dilate_tile_kernel<3>), dim3((unsigned)ntiles), dim3(kDTB), 0, stream, D, beg, end, nt, hx, hy,
+                           (int32_t)rpt, (uint32_t)ntiles, off_d, beg_out, end_out, capacity, d_total, status, scratch,
+                           epoch, stx_spin_limit());
+    else if (K <= 5)
+        hipLaunchKernelGGL((dilate_tile_kernel<5>), dim3((unsigned)ntiles), dim3(kDTB), 0, stream, D, beg, end, nt, hx, hy,
+                           (int32_t)rpt, (uint32_t)ntiles, off_d, beg_out, end_out, capacity, d_total, status, scratch,
+                           epoch, stx_spin_limit());
+    else
+        hipLaunchKernelGGL((dilate_tile_kernel<9>), dim3((unsigned)ntiles), dim3(kDTB), 0, stream, D, beg, end, nt, hx, hy,
+                           (int32_t)rpt, (uint32_t)ntiles, off_d, beg_out, end_out, capacity, d_total, status, scratch,
+                           epoch, stx_spin_limit());
+    rc = stx_launch_status("dilate_tile_kernel");
+    if (rc) return rc;
+    hipLaunchKernelGGL(dense_rows_kernel, dim3(nblk(nt)), dim3(kT), 0, stream, rows_d, nt, (int32_t)(row_lo - hy));
+    return stx_compact_rows(rows_d, off_d, nt, 0, rows_out, off_out, d_ny, ctmp, status, scratch, scratch_bytes,
+                            epoch + 1, stream);
 }
 
 }  // extern "C"

@@ -58,6 +58,23 @@ __global__ void rows_lookup_kernel(const int32_t* __restrict__ rows, int64_t R, 
     map[y] = (lo < R && rows[lo] == v) ? (int32_t)lo : -1;
 }
 
+// Dense offsets over the rows y0 .. y0 + n - 1: out[j] = off[k], k the first
+// row index with rows[k] >= y0 + j, i.e. the number of intervals (counted
+// from off[0]) in rows below y0 + j; rows the set does not hold come out
+// empty.  n + 1 entries.
+__global__ void dense_offsets_kernel(const int32_t* __restrict__ rows, const int32_t* __restrict__ off, int64_t R,
+                                     int64_t y0, int64_t n, int32_t* __restrict__ out) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j > n) return;
+    const int64_t y = y0 + j;
+    int64_t lo = 0, hi = R;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if ((int64_t)rows[mid] < y) lo = mid + 1; else hi = mid;
+    }
+    out[j] = off[lo];
+}
+
 __global__ void nonempty_flag_kernel(const int32_t* __restrict__ off, int64_t R, int32_t* __restrict__ flag) {
     int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r < R) flag[r] = off[r + 1] > off[r] ? 1 : 0;
@@ -222,6 +239,13 @@ int32_t stx_rows_lookup(const int32_t* rows, int64_t R, int32_t y0, int64_t H, i
     if (H <= 0) return STX_OK;
     hipLaunchKernelGGL(rows_lookup_kernel, dim3(nblk(H)), dim3(kT), 0, stream, rows, R, y0, H, map);
     return stx_launch_status("rows_lookup");
+}
+
+int32_t stx_dense_offsets(const int32_t* rows, const int32_t* off, int64_t R, int64_t y0, int64_t n, int32_t* out,
+                          hipStream_t stream) {
+    if (R < 0 || n < 0) return stx_fail(STX_EINVAL, "dense_offsets: negative size");
+    hipLaunchKernelGGL(dense_offsets_kernel, dim3(nblk(n + 1)), dim3(kT), 0, stream, rows, off, R, y0, n, out);
+    return stx_launch_status("dense_offsets");
 }
 
 // Drops the rows without intervals.  tmp: 2*(R+1) int32.  rows_out/off_out

@@ -149,8 +149,8 @@ def morphology_batched(batch: int, reps: int, dev) -> Dict:
     """Dilation by the square structuring element of radius 1, 2, 4 on
     ``batch`` set-algebra instances (4096 x 65536, the A sets of the batched
     set-algebra workload) stacked row-wise with 16 empty rows between them,
-    one dilation call for the whole stack (a six-launch fused pipeline with
-    one host read).  Algorithmic bytes: 8 B per input and output interval and
+    one dilation call for the whole stack (dense offsets, the tiled LDS
+    window union, row compaction: one host read).  Algorithmic bytes: 8 B per input and output interval and
     4 B per input and output row id and offset."""
     import subsetix_amd as sx
 

@@ -376,8 +376,8 @@ def test_stencil_schedules_vs_oracle(sx, schedule):
 def test_dilate_windows_vs_oracle(sx, hy):
     """Dilation on sparse rows with gaps, empty rows kept in the input,
     negative row ids and touching intervals, hx 0..3, against the oracle:
-    through dilate_interval_set (fused kernel for hy <= 2, row doubling
-    above) and through the fused kernel directly for every hy it takes."""
+    through dilate_interval_set and through the tiled dense-row kernel and
+    the per-row fused kernel directly for every hy they take."""
     rng = np.random.default_rng(40 + hy)
     rows = np.sort(rng.choice(np.arange(-30, 200), 90, replace=False))
     s = random_canonical(rng, 90, 300, mean_n=3, rows=rows, touching=True)
@@ -392,11 +392,69 @@ def test_dilate_windows_vs_oracle(sx, hy):
         noff.append(len(beg))
     s2 = ox.make(noff, beg, end, s.rows)
     from subsetix_amd import _lib
-    from subsetix_amd.morphology import _dilate_fused
+    from subsetix_amd.morphology import _dilate_fused, _dilate_tiled
 
+    lo, hi = int(s2.rows[0]), int(s2.rows[-1])
     for hx in (0, 1, 3):
         print("dilate hy", hy, "hx", hx, flush=True)
         exp = ox.dilate(s2, hx, hy)
         assert_set_equal(to_host(sx.dilate_interval_set(to_device(s2), hx, hy)), exp)
         if hy <= _lib.lib().stx_dilate_max_halo():
             assert_set_equal(to_host(_dilate_fused(to_device(s2), hx, hy)), exp)
+            assert_set_equal(to_host(_dilate_tiled(to_device(s2), hx, hy, lo, hi)), exp)
+
+
+def _rows_set(rows, per_row):
+    off, beg, end = [0], [], []
+    for iv in per_row:
+        for b, e in iv:
+            beg.append(b)
+            end.append(e)
+        off.append(len(beg))
+    return ox.make(off, beg, end, np.asarray(rows, dtype=np.int32))
+
+
+@pytest.mark.parametrize("hy", [1, 2, 4])
+def test_dilate_tiles_edge_cases_vs_oracle(sx, hy):
+    """The tiled dense-row dilation where a tile leaves the LDS layout and
+    merges its rows one thread per row: a tile whose input exceeds an LDS
+    buffer (one row of 6000 intervals), doubling steps that outgrow a
+    buffer (rows of unit intervals at row-dependent offsets, so that every
+    union of shifted rows adds intervals), x coordinates spanning 2^30 (keys
+    would not fit), row ids next to the int32 limits; each against the
+    oracle, through dilate_interval_set and the tiled kernel directly."""
+    from subsetix_amd.morphology import _dilate_tiled
+
+    rng = np.random.default_rng(90 + hy)
+    cases = []
+    # one long row among ordinary ones
+    rows = np.arange(0, 300)
+    per = [[(b, b + 2) for b in range(0, 24000, 4)] if r == 150 else
+           [(int(b), int(b) + 3) for b in np.sort(rng.choice(np.arange(0, 4000, 5), 30, replace=False))]
+           for r in rows]
+    cases.append(("long row", rows, per))
+    # unit intervals at offsets that differ from row to row: unions of
+    # shifted rows grow by the row count
+    rows = np.arange(-50, 450)
+    per = [[(16 * k + 2 * (r % 8), 16 * k + 2 * (r % 8) + 1) for k in range(70)] for r in rows]
+    cases.append(("growing unions", rows, per))
+    # coordinates spanning 2^30
+    rows = np.arange(0, 200)
+    per = [[(-(1 << 29) + 7 * r, -(1 << 29) + 7 * r + 5), (3 * r, 3 * r + 2), ((1 << 29) - 20 - r, (1 << 29) - r)]
+           for r in rows]
+    cases.append(("wide x", rows, per))
+    # row ids next to either end of the int32 range
+    top = (1 << 31) - 1 - hy
+    for name, rows in (("int32 low rows", np.arange(-(1 << 31) + hy, -(1 << 31) + hy + 40)),
+                       ("int32 high rows", np.arange(top - 60, top + 1))):
+        per = [[(int(b), int(b) + 2) for b in np.sort(rng.choice(np.arange(0, 600, 3), 12, replace=False))]
+               for _ in rows]
+        cases.append((name, rows, per))
+    for name, rows, per in cases:
+        s = _rows_set(rows, per)
+        for hx in (0, 2):
+            print("tiles", name, "hy", hy, "hx", hx, flush=True)
+            exp = ox.dilate(s, hx, hy)
+            d = to_device(s)
+            assert_set_equal(to_host(sx.dilate_interval_set(d, hx, hy)), exp)
+            assert_set_equal(to_host(_dilate_tiled(d, hx, hy, int(s.rows[0]), int(s.rows[-1]))), exp)
