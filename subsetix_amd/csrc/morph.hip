@@ -179,17 +179,6 @@ static_assert(2 * kDCap == kDTB * kDVT, "a step merges a buffer with its suffix"
 __device__ __forceinline__ int32_t dsk(int32_t i) { return i + (i >> 4); }  // one spare slot per 16: fewer bank conflicts
 constexpr int dsk_size(int n) { return n + n / 16 + 1; }
 
-// First index in [0, n) whose key is >= key (n if none).
-__device__ __forceinline__ int32_t lds_lower_bound(const int2* X, int32_t n, int32_t key) {
-    int32_t lo = 0, hi = n;
-    while (lo < hi) {
-        const int32_t mid = (lo + hi) >> 1;
-        if (X[dsk(mid)].x < key) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo;
-}
-
 // Union of the dense input rows j = i .. i + K - 1, widened by hx: the
 // interval count (WRITE = false) or the intervals written from ob/oe + w.
 template <int KMAX, bool WRITE>
@@ -272,17 +261,28 @@ __global__ __launch_bounds__(kDTB) __attribute__((amdgpu_waves_per_eu(8))) void 
     const int32_t nIn = D[i0 + Rt] - base;
     bool lds_ok = nIn <= kDCap;  // uniform
     for (int r = tid; r <= Rt; r += kDTB) s_rs[r] = D[i0 + r] - base;
+    // this thread's intervals: kPerThread consecutive ones from q0
+    const int32_t q0 = tid * kPerThread;
+    const int32_t n_mine = lds_ok ? (nIn - q0 < 0 ? 0 : (nIn - q0 > kPerThread ? kPerThread : nIn - q0)) : 0;
     int32_t xb[kPerThread], xe[kPerThread];
     int32_t lo = INT32_MAX, hi = INT32_MIN;
-    if (lds_ok) {
+    if (n_mine == kPerThread) {
+        int4 vb, ve;
+        __builtin_memcpy(&vb, __builtin_assume_aligned(beg + base + q0, 4), 16);
+        __builtin_memcpy(&ve, __builtin_assume_aligned(end + base + q0, 4), 16);
+        xb[0] = vb.x, xb[1] = vb.y, xb[2] = vb.z, xb[3] = vb.w;
+        xe[0] = ve.x, xe[1] = ve.y, xe[2] = ve.z, xe[3] = ve.w;
+    } else {
 #pragma unroll
         for (int u = 0; u < kPerThread; ++u) {
-            const int32_t q = tid + u * kDTB;
-            xb[u] = q < nIn ? beg[base + q] : INT32_MAX;
-            xe[u] = q < nIn ? end[base + q] : INT32_MIN;
-            lo = imin(lo, xb[u]);
-            hi = imax(hi, xe[u]);
+            xb[u] = u < n_mine ? beg[base + q0 + u] : INT32_MAX;
+            xe[u] = u < n_mine ? end[base + q0 + u] : INT32_MIN;
         }
+    }
+#pragma unroll
+    for (int u = 0; u < kPerThread; ++u) {
+        lo = imin(lo, xb[u]);
+        hi = imax(hi, xe[u]);
     }
     lo = wave_last(wave_incl_min(lo));
     hi = wave_last(wave_incl_max(hi));
@@ -310,19 +310,23 @@ __global__ __launch_bounds__(kDTB) __attribute__((amdgpu_waves_per_eu(8))) void 
     int cur = 0;
     if (lds_ok) {
         // stage: interval q of local row k -> key (k << shift) + (x - xmin)
+        if (n_mine > 0) {
+            int k = 0, kh = Rt;  // s_rs[k] <= q0 < s_rs[k + 1]
+            while (kh - k > 1) {
+                const int mid = (k + kh) >> 1;
+                if (s_rs[mid] <= q0) k = mid;
+                else kh = mid;
+            }
+            int32_t nxt = s_rs[k + 1];
 #pragma unroll
-        for (int u = 0; u < kPerThread; ++u) {
-            const int32_t q = tid + u * kDTB;
-            if (q < nIn) {
-                int k = 0, kh = Rt;  // s_rs[k] <= q < s_rs[k + 1]
-                while (kh - k > 1) {
-                    const int mid = (k + kh) >> 1;
-                    if (s_rs[mid] <= q) k = mid;
-                    else kh = mid;
+            for (int u = 0; u < kPerThread; ++u) {
+                const int32_t q = q0 + u;
+                if (u < n_mine) {
+                    while (q >= nxt) nxt = s_rs[++k + 1];  // s_rs[Rt] = nIn > q
+                    const uint32_t off = ((uint32_t)k << shift) - (uint32_t)xmin;
+                    s_buf[0][dsk(q)] = make_int2((int32_t)((uint32_t)xb[u] - (uint32_t)hx + off),
+                                                 (int32_t)((uint32_t)xe[u] + (uint32_t)hx + off));
                 }
-                const uint32_t off = ((uint32_t)k << shift) - (uint32_t)xmin;
-                s_buf[0][dsk(q)] = make_int2((int32_t)((uint32_t)xb[u] - (uint32_t)hx + off),
-                                             (int32_t)((uint32_t)xe[u] + (uint32_t)hx + off));
             }
         }
         __syncthreads();  // staged keys
@@ -335,9 +339,11 @@ __global__ __launch_bounds__(kDTB) __attribute__((amdgpu_waves_per_eu(8))) void 
             const int2* X = s_buf[cur];
             int2* Y = s_buf[cur ^ 1];
             const int32_t delta = (int32_t)((uint32_t)dq << shift);
-            const int32_t nA = last ? lds_lower_bound(X, nX, (int32_t)((uint32_t)nr << shift)) : nX;
-            const int32_t b0 = lds_lower_bound(X, nX, delta);
-            const int32_t b1 = last ? lds_lower_bound(X, nX, (int32_t)((uint32_t)(nr + dq) << shift)) : nX;
+            // s_rs: starts of X's local rows 0 .. Rt (read here, rewritten
+            // for Y once the scans' barriers have passed)
+            const int32_t nA = last ? s_rs[nr] : nX;
+            const int32_t b0 = s_rs[dq];
+            const int32_t b1 = last ? s_rs[nr + dq] : nX;
             const int32_t nB = b1 - b0;
             const int32_t P = nA + nB;
             const int32_t p0 = tid * kDVT;
@@ -404,18 +410,33 @@ __global__ __launch_bounds__(kDTB) __attribute__((amdgpu_waves_per_eu(8))) void 
                 lds_ok = false;
                 break;
             }
+            // outputs, and Y's row starts: a row's first position always
+            // starts a run (rows never touch), so the row of the last
+            // output before this thread is that of the largest end before
+            // it, the carry; rows (previous output's row, r] start at the
+            // output of row r
             int32_t w = my_base;
+            int32_t prow = carry >= 0 ? carry >> shift : -1;
 #pragma unroll
             for (int v = 0; v < kDVT; ++v) {
                 if (emit & (1u << v)) {
                     Y[dsk(w)].x = rb[v];
                     if (w > 0) Y[dsk(w - 1)].y = imax(carry, re[v]);
+                    const int32_t r = rb[v] >> shift;
+                    for (int32_t rr = prow + 1; rr <= r; ++rr) s_rs[rr] = w;
+                    prow = r;
                     ++w;
                 }
             }
-            // the owner of the last position closes the last run
-            if (step_total > 0 && p0 < P && P - p0 <= kDVT) Y[dsk(step_total - 1)].y = imax(carry, lm);
-            __syncthreads();  // Y complete; every read of X done
+            if (step_total == 0) {
+                for (int r = tid; r <= Rt; r += kDTB) s_rs[r] = 0;
+            } else if (p0 < P && P - p0 <= kDVT) {
+                // the owner of the last position closes the last run; the
+                // rows after the last output's start at the end
+                Y[dsk(step_total - 1)].y = imax(carry, lm);
+                for (int32_t rr = (lm >> shift) + 1; rr <= Rt; ++rr) s_rs[rr] = step_total;  // lm: in the last row
+            }
+            __syncthreads();  // Y and its row starts complete; every read of X done
             cur ^= 1;
             nX = step_total;
             if (last) break;
@@ -440,20 +461,15 @@ __global__ __launch_bounds__(kDTB) __attribute__((amdgpu_waves_per_eu(8))) void 
     if (over && tid == 0 && total > 0) atomicOr(status, STX_FLAG_CAPACITY);
     if (lds_ok) {
         const int2* X = s_buf[cur];
-        for (int32_t k = tid; k < total; k += kDTB) {
-            const int2 o = X[dsk(k)];
-            const int32_t r = o.x >> shift;
-            if (!over) {
-                const uint32_t boff = ((uint32_t)r << shift) - (uint32_t)xmin;
+        if (!over) {
+            for (int32_t k = tid; k < total; k += kDTB) {
+                const int2 o = X[dsk(k)];
+                const uint32_t boff = ((uint32_t)(o.x >> shift) << shift) - (uint32_t)xmin;
                 ob[excl + k] = (int32_t)((uint32_t)o.x - boff);
                 oe[excl + k] = (int32_t)((uint32_t)o.y - boff);
             }
-            // rows (row of output k - 1, row of output k] start at output k
-            const int32_t rp = k > 0 ? (X[dsk(k - 1)].x >> shift) : -1;
-            for (int32_t rr = rp + 1; rr <= r; ++rr) out_off[i0 + rr] = (int32_t)(excl + k);
         }
-        const int32_t rl = total > 0 ? (X[dsk(total - 1)].x >> shift) : -1;
-        for (int32_t rr = rl + 1 + tid; rr < nr; rr += kDTB) out_off[i0 + rr] = (int32_t)(excl + total);
+        for (int r = tid; r < nr; r += kDTB) out_off[i0 + r] = (int32_t)(excl + s_rs[r]);
     } else if (tid < nr) {
         out_off[i0 + tid] = (int32_t)(excl + ex);
         if (!over) dense_window_union<KMAX, true>(D, beg, end, i0 + tid, K, hx, ob, oe, excl + ex, capacity, status);
