@@ -236,6 +236,129 @@ __device__ int32_t dense_window_union(const int32_t* __restrict__ D, const int32
     return n;
 }
 
+// One doubling step, called by every thread of the block: Y = X[0, nA) U
+// (X[b0, b1) lowered by dq rows), compacted, with the row starts in rs
+// (those of X on entry, read by the caller before the call) rewritten for Y
+// over the local rows 0 .. Rt.  Returns Y's count, or -1 when it would not
+// fit a buffer (uniform; Y and rs are then left as they are).
+__device__ __forceinline__ int32_t union_step(const int2* __restrict__ X, int2* __restrict__ Y, int32_t* rs,
+                                              int32_t nA, int32_t b0, int32_t b1, int dq, uint32_t shift, int Rt,
+                                              int32_t* wmax, int32_t* wsum) {
+    constexpr int kTW = kDTB / STX_WAVE;
+    const int tid = threadIdx.x;
+    const int32_t delta = (int32_t)((uint32_t)dq << shift);
+    const int32_t nB = b1 - b0;
+    const int32_t P = nA + nB;
+    const int32_t p0 = tid * kDVT;
+    const uint32_t valid = p0 >= P ? 0u : (P - p0 >= kDVT ? (1u << kDVT) - 1u : (1u << (P - p0)) - 1u);
+    int32_t rb[kDVT], re[kDVT];
+    uint32_t emit = 0;
+    int32_t lm = INT32_MIN;  // largest end key before the current position (this thread's part)
+    {
+        int32_t ia = nA, jb = nB;  // positions past the end read sentinels only
+        if (p0 < P) {
+            int32_t l2 = p0 - nB > 0 ? p0 - nB : 0, h2 = p0 < nA ? p0 : nA;
+            while (l2 < h2) {
+                const int32_t mid = (l2 + h2) >> 1;
+                if (X[dsk(mid)].x <= X[dsk(b0 + p0 - 1 - mid)].x - delta) l2 = mid + 1;
+                else h2 = mid;
+            }
+            ia = l2;
+            jb = p0 - l2;
+        }
+        int2 ca = ia < nA ? X[dsk(ia)] : make_int2(INT32_MAX, INT32_MIN);
+        int2 cb = make_int2(INT32_MAX, INT32_MIN);
+        if (jb < nB) {
+            cb = X[dsk(b0 + jb)];
+            cb.x -= delta;
+            cb.y -= delta;
+        }
+#pragma unroll
+        for (int v = 0; v < kDVT; ++v) {
+            const bool isA = ca.x <= cb.x;
+            const int32_t x = isA ? ca.x : cb.x;
+            const int32_t y = isA ? ca.y : cb.y;
+            rb[v] = x;
+            re[v] = lm;
+            emit |= (x > lm ? 1u : 0u) << v;
+            lm = imax(lm, y);
+            if (isA) {
+                ++ia;
+                ca = ia < nA ? X[dsk(ia)] : make_int2(INT32_MAX, INT32_MIN);
+            } else {
+                ++jb;
+                if (jb < nB) {
+                    cb = X[dsk(b0 + jb)];
+                    cb.x -= delta;
+                    cb.y -= delta;
+                } else {
+                    cb = make_int2(INT32_MAX, INT32_MIN);
+                }
+            }
+        }
+    }
+    // a position starts a run when its begin is above every end before it;
+    // begins grow along the walk, so those at or below the carry form a
+    // prefix of the thread's positions
+    const int32_t carry = block_excl_max<kTW>(lm, wmax);
+    int32_t below = 0;
+#pragma unroll
+    for (int v = 0; v < kDVT; ++v) below += rb[v] <= carry ? 1 : 0;
+    emit &= ~((1u << below) - 1u);
+    emit &= valid;
+    int32_t step_total = 0;
+    const int32_t my_base = block_excl_sum<kTW>(__popc(emit), wsum, &step_total);
+    if (step_total > kDCap) return -1;
+    // outputs, and Y's row starts: a row's first position always starts a
+    // run (rows never touch), so the row of the last output before this
+    // thread is that of the largest end before it, the carry; rows
+    // (previous output's row, r] start at the output of row r
+    int32_t w = my_base;
+    int32_t prow = carry >= 0 ? carry >> shift : -1;
+#pragma unroll
+    for (int v = 0; v < kDVT; ++v) {
+        if (emit & (1u << v)) {
+            Y[dsk(w)].x = rb[v];
+            if (w > 0) Y[dsk(w - 1)].y = imax(carry, re[v]);
+            const int32_t r = rb[v] >> shift;
+            for (int32_t rr = prow + 1; rr <= r; ++rr) rs[rr] = w;
+            prow = r;
+            ++w;
+        }
+    }
+    if (step_total == 0) {
+        for (int r = tid; r <= Rt; r += kDTB) rs[r] = 0;
+    } else if (p0 < P && P - p0 <= kDVT) {
+        // the owner of the last position closes the last run; the rows after
+        // the last output's start at the end (lm lies in the last row)
+        Y[dsk(step_total - 1)].y = imax(carry, lm);
+        for (int32_t rr = (lm >> shift) + 1; rr <= Rt; ++rr) rs[rr] = step_total;
+    }
+    __syncthreads();  // Y and its row starts complete; every read of X done
+    return step_total;
+}
+
+// The window union of a staged tile: doubling steps W2q(l) = Wq(l) U
+// Wq(l + q) while 2q <= K, then out(l) = Wp(l) U Wp(l + K - p), l < nr.
+// Returns the count (the result in buf[*cur]) or -1 on overflow.
+__device__ __forceinline__ int32_t window_union_steps(int2 (*buf)[dsk_size(kDCap)], int& cur, int32_t* rs, int32_t nX,
+                                                      int K, int nr, int Rt, uint32_t shift, int32_t* wmax,
+                                                      int32_t* wsum) {
+    int p = 1;
+    while (2 * p <= K) p *= 2;
+    for (int q = 1;; q *= 2) {
+        const bool last = q >= p;
+        const int dq = last ? K - p : q;  // rows the suffix is lowered by
+        const int32_t nA = last ? rs[nr] : nX;
+        const int32_t b0 = rs[dq];
+        const int32_t b1 = last ? rs[nr + dq] : nX;
+        nX = union_step(buf[cur], buf[cur ^ 1], rs, nA, b0, b1, dq, shift, Rt, wmax, wsum);
+        if (nX < 0) return -1;
+        cur ^= 1;
+        if (last) return nX;
+    }
+}
+
 template <int KMAX>
 __global__ __launch_bounds__(kDTB) __attribute__((amdgpu_waves_per_eu(8))) void dilate_tile_kernel(
     const int32_t* __restrict__ D, const int32_t* __restrict__ beg, const int32_t* __restrict__ end, int64_t nt,
@@ -330,118 +453,8 @@ __global__ __launch_bounds__(kDTB) __attribute__((amdgpu_waves_per_eu(8))) void 
             }
         }
         __syncthreads();  // staged keys
-        int p = 1;
-        while (2 * p <= K) p *= 2;
-        int32_t nX = nIn;
-        for (int q = 1;; q *= 2) {
-            const bool last = q >= p;
-            const int dq = last ? K - p : q;  // rows the suffix is lowered by
-            const int2* X = s_buf[cur];
-            int2* Y = s_buf[cur ^ 1];
-            const int32_t delta = (int32_t)((uint32_t)dq << shift);
-            // s_rs: starts of X's local rows 0 .. Rt (read here, rewritten
-            // for Y once the scans' barriers have passed)
-            const int32_t nA = last ? s_rs[nr] : nX;
-            const int32_t b0 = s_rs[dq];
-            const int32_t b1 = last ? s_rs[nr + dq] : nX;
-            const int32_t nB = b1 - b0;
-            const int32_t P = nA + nB;
-            const int32_t p0 = tid * kDVT;
-            const uint32_t valid =
-                p0 >= P ? 0u : (P - p0 >= kDVT ? (1u << kDVT) - 1u : (1u << (P - p0)) - 1u);
-            int32_t rb[kDVT], re[kDVT];
-            uint32_t emit = 0;
-            int32_t lm = INT32_MIN;  // largest end key before the current position (this thread's part)
-            {
-                int32_t ia = nA, jb = nB;  // positions past the end read sentinels only
-                if (p0 < P) {
-                    int32_t l2 = p0 - nB > 0 ? p0 - nB : 0, h2 = p0 < nA ? p0 : nA;
-                    while (l2 < h2) {
-                        const int32_t mid = (l2 + h2) >> 1;
-                        if (X[dsk(mid)].x <= X[dsk(b0 + p0 - 1 - mid)].x - delta) l2 = mid + 1;
-                        else h2 = mid;
-                    }
-                    ia = l2;
-                    jb = p0 - l2;
-                }
-                int2 ca = ia < nA ? X[dsk(ia)] : make_int2(INT32_MAX, INT32_MIN);
-                int2 cb = make_int2(INT32_MAX, INT32_MIN);
-                if (jb < nB) {
-                    cb = X[dsk(b0 + jb)];
-                    cb.x -= delta;
-                    cb.y -= delta;
-                }
-#pragma unroll
-                for (int v = 0; v < kDVT; ++v) {
-                    const bool isA = ca.x <= cb.x;
-                    const int32_t x = isA ? ca.x : cb.x;
-                    const int32_t y = isA ? ca.y : cb.y;
-                    rb[v] = x;
-                    re[v] = lm;
-                    emit |= (x > lm ? 1u : 0u) << v;
-                    lm = imax(lm, y);
-                    if (isA) {
-                        ++ia;
-                        ca = ia < nA ? X[dsk(ia)] : make_int2(INT32_MAX, INT32_MIN);
-                    } else {
-                        ++jb;
-                        if (jb < nB) {
-                            cb = X[dsk(b0 + jb)];
-                            cb.x -= delta;
-                            cb.y -= delta;
-                        } else {
-                            cb = make_int2(INT32_MAX, INT32_MIN);
-                        }
-                    }
-                }
-            }
-            // a position starts a run when its begin is above every end
-            // before it; begins grow along the walk, so those at or below
-            // the carry form a prefix of the thread's positions
-            const int32_t carry = block_excl_max<kTW>(lm, s_wmax);
-            int32_t below = 0;
-#pragma unroll
-            for (int v = 0; v < kDVT; ++v) below += rb[v] <= carry ? 1 : 0;
-            emit &= ~((1u << below) - 1u);
-            emit &= valid;
-            int32_t step_total = 0;
-            const int32_t my_base = block_excl_sum<kTW>(__popc(emit), s_wsum, &step_total);
-            if (step_total > kDCap) {  // uniform
-                lds_ok = false;
-                break;
-            }
-            // outputs, and Y's row starts: a row's first position always
-            // starts a run (rows never touch), so the row of the last
-            // output before this thread is that of the largest end before
-            // it, the carry; rows (previous output's row, r] start at the
-            // output of row r
-            int32_t w = my_base;
-            int32_t prow = carry >= 0 ? carry >> shift : -1;
-#pragma unroll
-            for (int v = 0; v < kDVT; ++v) {
-                if (emit & (1u << v)) {
-                    Y[dsk(w)].x = rb[v];
-                    if (w > 0) Y[dsk(w - 1)].y = imax(carry, re[v]);
-                    const int32_t r = rb[v] >> shift;
-                    for (int32_t rr = prow + 1; rr <= r; ++rr) s_rs[rr] = w;
-                    prow = r;
-                    ++w;
-                }
-            }
-            if (step_total == 0) {
-                for (int r = tid; r <= Rt; r += kDTB) s_rs[r] = 0;
-            } else if (p0 < P && P - p0 <= kDVT) {
-                // the owner of the last position closes the last run; the
-                // rows after the last output's start at the end
-                Y[dsk(step_total - 1)].y = imax(carry, lm);
-                for (int32_t rr = (lm >> shift) + 1; rr <= Rt; ++rr) s_rs[rr] = step_total;  // lm: in the last row
-            }
-            __syncthreads();  // Y and its row starts complete; every read of X done
-            cur ^= 1;
-            nX = step_total;
-            if (last) break;
-        }
-        total = nX;
+        total = window_union_steps(s_buf, cur, s_rs, nIn, K, nr, Rt, shift, s_wmax, s_wsum);
+        lds_ok = total >= 0;
     }
     int32_t ex = 0;  // row-merge path: this thread's row's exclusive output count
     if (!lds_ok) {
@@ -476,6 +489,320 @@ __global__ __launch_bounds__(kDTB) __attribute__((amdgpu_waves_per_eu(8))) void 
     }
     if (i0 + nr == nt && tid == 0) {
         out_off[nt] = (int32_t)(excl + total);
+        if (d_total) *d_total = (int32_t)(excl + total);
+    }
+}
+
+// One complement step, called by every thread of the block: for each local
+// row r < Rc of X (rows starting at rs[r]), the gaps between the runs of its
+// non-empty intervals inside the row's band [0, W) (keys (r << shift) + hx
+// .. (r << shift) + W + hx), each widened by hw, in order, into Y; only rows
+// in [ok_lo, ok_hi) produce gaps.  Slots: row r's intervals k at k + r, then
+// its row-end slot at rs[r + 1] + r (the gap after the row's last run).  A
+// gap ends at an interval's begin (or the band's end) and starts at the
+// largest end before it (or the band's start): the thread's own part of
+// that maximum is kept per slot and the part before the thread (the carry)
+// folded in after one block max.  Rewrites rs for Y (rows 0 .. Rc).
+// Returns Y's count or -1 when it would not fit a buffer (uniform).
+__device__ __forceinline__ int32_t complement_step(const int2* __restrict__ X, int2* __restrict__ Y, int32_t* rs,
+                                                   int Rc, int ok_lo, int ok_hi, int32_t W, int32_t hx, int32_t hw,
+                                                   uint32_t shift, int32_t* wmax, int32_t* wsum) {
+    constexpr int kTW = kDTB / STX_WAVE;
+    const int tid = threadIdx.x;
+    const int32_t P = rs[Rc] + Rc;
+    const int32_t p0 = tid * kDVT;
+    int32_t gs[kDVT], g1[kDVT];  // gap start (own part of the maximum) and end, keys
+    uint32_t cand = 0, first = 0, rowend = 0;
+    int32_t lm = INT32_MIN;  // largest end key of this thread's non-empty intervals so far
+    int r0 = 0;
+    if (p0 < P) {
+        int k = 0, kh = Rc;  // row of slot p0: rs[k] + k <= p0 < rs[k + 1] + k + 1
+        while (kh - k > 1) {
+            const int mid = (k + kh) >> 1;
+            if (rs[mid] + mid <= p0) k = mid;
+            else kh = mid;
+        }
+        r0 = k;
+    }
+    {
+        int r = r0;
+        int32_t rstart = rs[r] + r, rnext = rs[r + 1];
+#pragma unroll
+        for (int v = 0; v < kDVT; ++v) {
+            const int32_t p = p0 + v;
+            const int32_t lo_key = (int32_t)(((uint32_t)r << shift) + (uint32_t)hx);
+            const int32_t hi_key = lo_key + W;
+            const bool ok = r >= ok_lo && r < ok_hi;
+            gs[v] = imax(lm, lo_key);
+            g1[v] = hi_key;
+            if (p < P) {
+                first |= (p == rstart ? 1u : 0u) << v;
+                if (p - r < rnext) {  // interval slot
+                    const int2 it = X[dsk(p - r)];
+                    if (it.y > it.x) {
+                        g1[v] = imin(it.x, hi_key);
+                        cand |= (ok ? 1u : 0u) << v;
+                        lm = imax(lm, it.y);
+                    }
+                } else {  // row-end slot
+                    cand |= (ok ? 1u : 0u) << v;
+                    rowend |= 1u << v;
+                    ++r;
+                    rstart = p + 1;
+                    rnext = r < Rc ? rs[r + 1] : 0;
+                }
+            }
+        }
+    }
+    const int32_t carry = block_excl_max<kTW>(lm, wmax);
+    uint32_t emit = 0;
+#pragma unroll
+    for (int v = 0; v < kDVT; ++v) {
+        gs[v] = imax(gs[v], carry);
+        emit |= ((cand & (1u << v)) && g1[v] > gs[v] ? 1u : 0u) << v;
+    }
+    int32_t step_total = 0;
+    const int32_t my_base = block_excl_sum<kTW>(__popc(emit), wsum, &step_total);
+    if (step_total > kDCap) return -1;
+    int32_t w = my_base;
+    int r = r0;
+#pragma unroll
+    for (int v = 0; v < kDVT; ++v) {
+        if (first & (1u << v)) rs[r] = w;
+        if (emit & (1u << v)) {
+            Y[dsk(w)] = make_int2(gs[v] - hw, g1[v] + hw);
+            ++w;
+        }
+        if (rowend & (1u << v)) ++r;
+    }
+    if (tid == 0) rs[Rc] = step_total;
+    __syncthreads();  // Y and its row starts complete; every read of X done
+    return step_total;
+}
+
+// Next non-empty gap inside [0, W) of one input row (interval cursor jj
+// before jje, pe the largest clipped end so far; -1 once the gap after the
+// row's last interval has been produced): [gb, g1), or gb = INT_MAX.
+__device__ __forceinline__ void next_gap(const int32_t* __restrict__ beg, const int32_t* __restrict__ end, int32_t& jj,
+                                         int32_t jje, int32_t& pe, int32_t W, int32_t& gb, int32_t& g1) {
+    while (true) {
+        if (jj < jje) {
+            const int32_t b = imin(imax(beg[jj], 0), W), e = imin(imax(end[jj], 0), W);
+            ++jj;
+            if (e > b) {
+                const bool gap = b > pe;
+                const int32_t s = pe;
+                pe = imax(pe, e);
+                if (gap) {
+                    gb = s;
+                    g1 = b;
+                    return;
+                }
+            }
+        } else if (pe >= 0) {
+            const int32_t s = pe;
+            pe = -1;
+            if (W > s) {
+                gb = s;
+                g1 = W;
+                return;
+            }
+        } else {
+            gb = INT_MAX;
+            return;
+        }
+    }
+}
+
+// Erosion of output row i (0 <= i < H), one thread: [0, W) minus the union
+// of the gaps inside [0, W) of the rows i - hy .. i + hy that lie in [0, H)
+// (dense rows j = i .. i + K - 1 of D, row j holding row j - hy), each gap
+// widened by hx.  The count (WRITE = false) or the intervals from ob/oe + w.
+template <int KMAX, bool WRITE>
+__device__ int32_t erode_row(const int32_t* __restrict__ D, const int32_t* __restrict__ beg,
+                             const int32_t* __restrict__ end, int64_t i, int K, int64_t H, int32_t hy, int32_t W,
+                             int32_t hx, int32_t* __restrict__ ob, int32_t* __restrict__ oe, int64_t w,
+                             int64_t capacity, uint32_t* __restrict__ status) {
+    int32_t j[KMAX], je[KMAX], pe[KMAX], gb[KMAX], g1[KMAX];
+#pragma unroll
+    for (int t = 0; t < KMAX; ++t) {
+        const int64_t y = i + t - hy;
+        const bool in = t < K && y >= 0 && y < H;
+        j[t] = in ? D[i + t] : 0;
+        je[t] = in ? D[i + t + 1] : 0;
+        pe[t] = in ? 0 : -1;
+        gb[t] = INT_MAX;
+        g1[t] = 0;
+        if (in) next_gap(beg, end, j[t], je[t], pe[t], W, gb[t], g1[t]);
+    }
+    int32_t n = 0, op = 0;  // op: where the next output piece may start
+    int32_t s = 0, m = 0;
+    bool open = false;
+    while (true) {
+        int32_t best = INT_MAX, tb = -1;
+#pragma unroll
+        for (int t = 0; t < KMAX; ++t)
+            if (gb[t] < best) { best = gb[t]; tb = t; }
+        int32_t b = 0, e = 0;
+        if (tb >= 0) {
+#pragma unroll
+            for (int t = 0; t < KMAX; ++t) {
+                if (t == tb) {
+                    b = gb[t] - hx;
+                    e = g1[t] + hx;
+                    next_gap(beg, end, j[t], je[t], pe[t], W, gb[t], g1[t]);
+                }
+            }
+        }
+        if (open && (tb < 0 || b > m)) {
+            // run [s, m) of the union closes: the piece of [0, W) before it
+            const int32_t cs = imin(imax(s, 0), W);
+            if (cs > op) {
+                if (WRITE) {
+                    if (w < capacity) { ob[w] = op; oe[w] = cs; } else atomicOr(status, STX_FLAG_CAPACITY);
+                    ++w;
+                }
+                ++n;
+            }
+            op = imax(op, imin(imax(m, 0), W));
+            open = false;
+        }
+        if (tb < 0) break;
+        if (!open) {
+            s = b;
+            m = e;
+            open = true;
+        } else if (e > m) {
+            m = e;
+        }
+    }
+    if (W > op) {
+        if (WRITE) {
+            if (w < capacity) { ob[w] = op; oe[w] = W; } else atomicOr(status, STX_FLAG_CAPACITY);
+        }
+        ++n;
+    }
+    return n;
+}
+
+// Erosion over the rows 0 .. H - 1 (erode_interval_set): U - clip(dilate(U -
+// clip(S))), U = [0, W) x [0, H), in one pass per tile of output rows.  A tile
+// stages its rows' input window (rows i0 - hy .. i0 + nr + hy; values
+// clipped to [0, W], keys (row << shift) + x + hx with 2^shift > W + 2 hx),
+// takes the complement inside [0, W) of the rows that lie in [0, H) with its
+// gaps widened by hx (complement_step), the window union by doubling
+// (window_union_steps), and the complement of that inside [0, W) (the
+// erosion).  A tile that leaves the LDS layout erodes each row in one thread
+// (erode_row).  Output: offsets of every row 0 .. H - 1.
+template <int KMAX>
+__global__ __launch_bounds__(kDTB) __attribute__((amdgpu_waves_per_eu(8))) void erode_tile_kernel(
+    const int32_t* __restrict__ D, const int32_t* __restrict__ beg, const int32_t* __restrict__ end, int64_t H,
+    int32_t W, int32_t hx, int32_t hy, uint32_t shift, int32_t rpt, uint32_t ntiles, int32_t* __restrict__ out_off,
+    int32_t* __restrict__ ob, int32_t* __restrict__ oe, int64_t capacity, int32_t* __restrict__ d_total,
+    uint32_t* status, void* scratch, uint32_t epoch, int64_t spin_limit) {
+    constexpr int kTW = kDTB / STX_WAVE;
+    constexpr int kPerThread = kDCap / kDTB;
+    __shared__ __align__(16) int2 s_buf[2][dsk_size(kDCap)];
+    __shared__ int32_t s_rs[kDRows];
+    __shared__ int32_t s_wmax[kTW];
+    __shared__ int32_t s_wsum[kTW];
+    __shared__ int32_t s_fb[kTW];
+    __shared__ uint32_t s_excl;
+    LookbackScratch lb = lookback_scratch(scratch, ntiles);
+    const uint32_t tile = blockIdx.x;
+    const int tid = threadIdx.x, lane = lane_id(), wave = wave_id();
+    const int K = 2 * hy + 1;
+    const int64_t i0 = (int64_t)tile * rpt;
+    const int nr = (int)(H - i0 < rpt ? H - i0 : rpt);  // ntiles = ceil(H / rpt): nr >= 1
+    const int Rt = nr + 2 * hy;                           // staged rows i0 - hy .. i0 + nr + hy - 1
+    const int32_t base = D[i0];
+    const int32_t nIn = D[i0 + Rt] - base;
+    bool lds_ok = nIn <= kDCap && ((int64_t)Rt << shift) < (1ll << 30);  // uniform
+    for (int r = tid; r <= Rt; r += kDTB) s_rs[r] = D[i0 + r] - base;
+    const int32_t q0 = tid * kPerThread;
+    const int32_t n_mine = lds_ok ? (nIn - q0 < 0 ? 0 : (nIn - q0 > kPerThread ? kPerThread : nIn - q0)) : 0;
+    int32_t xb[kPerThread], xe[kPerThread];
+    if (n_mine == kPerThread) {
+        int4 vb, ve;
+        __builtin_memcpy(&vb, __builtin_assume_aligned(beg + base + q0, 4), 16);
+        __builtin_memcpy(&ve, __builtin_assume_aligned(end + base + q0, 4), 16);
+        xb[0] = vb.x, xb[1] = vb.y, xb[2] = vb.z, xb[3] = vb.w;
+        xe[0] = ve.x, xe[1] = ve.y, xe[2] = ve.z, xe[3] = ve.w;
+    } else {
+#pragma unroll
+        for (int u = 0; u < kPerThread; ++u) {
+            xb[u] = u < n_mine ? beg[base + q0 + u] : 0;
+            xe[u] = u < n_mine ? end[base + q0 + u] : 0;
+        }
+    }
+    __syncthreads();  // row starts
+    int32_t total = 0;
+    int cur = 0;
+    if (lds_ok) {
+        if (n_mine > 0) {
+            int k = 0, kh = Rt;  // s_rs[k] <= q0 < s_rs[k + 1]
+            while (kh - k > 1) {
+                const int mid = (k + kh) >> 1;
+                if (s_rs[mid] <= q0) k = mid;
+                else kh = mid;
+            }
+            int32_t nxt = s_rs[k + 1];
+#pragma unroll
+            for (int u = 0; u < kPerThread; ++u) {
+                const int32_t q = q0 + u;
+                if (u < n_mine) {
+                    while (q >= nxt) nxt = s_rs[++k + 1];
+                    const int32_t off = (int32_t)(((uint32_t)k << shift) + (uint32_t)hx);
+                    s_buf[0][dsk(q)] = make_int2(imin(imax(xb[u], 0), W) + off, imin(imax(xe[u], 0), W) + off);
+                }
+            }
+        }
+        __syncthreads();  // staged keys
+        // local row r holds row i0 + r - hy: rows in [0, H) take part
+        const int ok_lo = (int)(hy - i0 > 0 ? hy - i0 : 0);
+        const int ok_hi = (int)(H + hy - i0 < Rt ? H + hy - i0 : Rt);
+        int32_t n = complement_step(s_buf[0], s_buf[1], s_rs, Rt, ok_lo, ok_hi, W, hx, hx, shift, s_wmax, s_wsum);
+        cur = 1;
+        if (n >= 0 && K > 1) n = window_union_steps(s_buf, cur, s_rs, n, K, nr, Rt, shift, s_wmax, s_wsum);
+        if (n >= 0) {
+            n = complement_step(s_buf[cur], s_buf[cur ^ 1], s_rs, nr, 0, nr, W, hx, 0, shift, s_wmax, s_wsum);
+            cur ^= 1;
+        }
+        lds_ok = n >= 0;
+        total = n;
+    }
+    int32_t ex = 0;  // row-merge path: this thread's row's exclusive output count
+    if (!lds_ok) {
+        const int32_t c = tid < nr ? erode_row<KMAX, false>(D, beg, end, i0 + tid, K, H, hy, W, hx, nullptr, nullptr,
+                                                            0, 0, status)
+                                   : 0;
+        ex = block_excl_sum<kTW>(c, s_fb, &total);
+    }
+    if (wave == 0) {
+        const uint32_t e = lookback_exclusive<1, false>(lb, tile, (uint32_t)total, epoch, status, spin_limit);
+        if (lane == 0) s_excl = e;
+    }
+    __syncthreads();
+    const int64_t excl = (int64_t)s_excl;
+    const bool over = excl + total > capacity;
+    if (over && tid == 0 && total > 0) atomicOr(status, STX_FLAG_CAPACITY);
+    if (lds_ok) {
+        const int2* X = s_buf[cur];
+        if (!over) {
+            for (int32_t k = tid; k < total; k += kDTB) {
+                const int2 o = X[dsk(k)];
+                const int32_t boff = (int32_t)(((uint32_t)(o.x >> shift) << shift) + (uint32_t)hx);
+                ob[excl + k] = o.x - boff;
+                oe[excl + k] = o.y - boff;
+            }
+        }
+        for (int r = tid; r < nr; r += kDTB) out_off[i0 + r] = (int32_t)(excl + s_rs[r]);
+    } else if (tid < nr) {
+        out_off[i0 + tid] = (int32_t)(excl + ex);
+        if (!over) erode_row<KMAX, true>(D, beg, end, i0 + tid, K, H, hy, W, hx, ob, oe, excl + ex, capacity, status);
+    }
+    if (i0 + nr == H && tid == 0) {
+        out_off[H] = (int32_t)(excl + total);
         if (d_total) *d_total = (int32_t)(excl + total);
     }
 }
@@ -600,6 +927,44 @@ int32_t stx_dilate_dense(const int32_t* rows, const int32_t* off, int64_t R, con
     hipLaunchKernelGGL(dense_rows_kernel, dim3(nblk(nt)), dim3(kT), 0, stream, rows_d, nt, (int32_t)(row_lo - hy));
     return stx_compact_rows(rows_d, off_d, nt, 0, rows_out, off_out, d_ny, ctmp, status, scratch, scratch_bytes,
                             epoch + 1, stream);
+}
+
+int64_t stx_erode_dense_tmp_ints(int64_t H, int32_t hy) { return H + 2 * (int64_t)hy + 1; }
+
+int64_t stx_erode_dense_scratch_bytes(int64_t H) { return lookback_bytes(H > 1 ? H : 1); }
+
+int32_t stx_erode_dense(const int32_t* rows, const int32_t* off, int64_t R, const int32_t* beg, const int32_t* end,
+                        int64_t N, int32_t hx, int32_t hy, int32_t W, int64_t H, int32_t* off_out, int32_t* beg_out,
+                        int32_t* end_out, int64_t capacity, int32_t* d_total, int32_t* tmp, int64_t tmp_ints,
+                        uint32_t* status, void* scratch, int64_t scratch_bytes, uint32_t epoch, hipStream_t stream) {
+    if (R < 0 || N < 0 || hx < 0 || hy < 0 || hy > kMaxHalo || W < 1 || H < 1 || H > INT32_MAX)
+        return stx_fail(STX_EINVAL, "erode_dense: need R, N, hx >= 0, 0 <= hy <= 4, W >= 1, 1 <= H < 2^31");
+    if (tmp_ints < stx_erode_dense_tmp_ints(H, hy)) return stx_fail(STX_EINVAL, "erode_dense: tmp too small");
+    if (scratch_bytes < stx_erode_dense_scratch_bytes(H)) return stx_fail(STX_EINVAL, "erode_dense: scratch too small");
+    int32_t* D = tmp;  // row j of D holds row j - hy, j = 0 .. H + 2 hy - 1
+    int32_t rc = stx_dense_offsets(rows, off, R, -(int64_t)hy, H + 2 * hy, D, stream);
+    if (rc) return rc;
+    const int64_t band = (int64_t)W + 2 * (int64_t)hx;  // key offsets 0 .. W + 2 hx
+    uint32_t shift = 0;
+    while (shift < 40 && ((int64_t)1 << shift) <= band) ++shift;
+    // the complement holds about one interval more per row than the set
+    constexpr double kDFill = 0.7;
+    const double mean = (double)N / (double)H + 1.0;
+    int64_t rpt = (int64_t)(kDFill * kDCap / mean) - 2 * hy;
+    if (rpt < 1) rpt = 1;
+    if (rpt > kDTB) rpt = kDTB;
+    const int64_t ntiles = (H + rpt - 1) / rpt;
+    const int K = 2 * hy + 1;
+#define STX_ERODE_LAUNCH(KM)                                                                                         \
+    hipLaunchKernelGGL((erode_tile_kernel<KM>), dim3((unsigned)ntiles), dim3(kDTB), 0, stream, D, beg, end, H, W, hx, \
+                       hy, shift, (int32_t)rpt, (uint32_t)ntiles, off_out, beg_out, end_out, capacity, d_total, status, \
+                       scratch, epoch, stx_spin_limit())
+    if (K <= 1) STX_ERODE_LAUNCH(1);
+    else if (K <= 3) STX_ERODE_LAUNCH(3);
+    else if (K <= 5) STX_ERODE_LAUNCH(5);
+    else STX_ERODE_LAUNCH(9);
+#undef STX_ERODE_LAUNCH
+    return stx_launch_status("erode_tile_kernel");
 }
 
 }  // extern "C"

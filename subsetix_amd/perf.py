@@ -146,30 +146,37 @@ def morphology(pair, reps: int, dev) -> Dict:
 
 
 def morphology_batched(batch: int, reps: int, dev) -> Dict:
-    """Dilation by the square structuring element of radius 1, 2, 4 on
+    """Dilation and erosion by the square structuring element of radius 1, 2, 4 on
     ``batch`` set-algebra instances (4096 x 65536, the A sets of the batched
     set-algebra workload) stacked row-wise with 16 empty rows between them,
     one dilation call for the whole stack (dense offsets, the tiled LDS
-    window union, row compaction: one host read).  Algorithmic bytes: 8 B per input and output interval and
+    window union, row compaction: one host read), and one erosion call over
+    the stack's bounding domain 65536 x (last row + 1) (one launch of row
+    tiles, one host read; the empty rows between instances are inside that
+    domain, so each instance also erodes from its top and bottom rows).  Algorithmic bytes: 8 B per input and output interval and
     4 B per input and output row id and offset."""
     import subsetix_amd as sx
 
     sets = [workloads.setalgebra_pair(seed_a=1 + 2 * k, seed_b=2 + 2 * k)[0] for k in range(batch)]
     h = workloads.stack_rows_gapped(sets, 16)
     s = _dev_set(h, dev, dense=False)
-    out = {"instances": batch, "rows": s.row_count, "input_intervals": s.interval_count}
+    height = int(h.rows[-1]) + 1
+    out = {"instances": batch, "rows": s.row_count, "input_intervals": s.interval_count,
+           "erode_domain": [65536, height]}
     for r in (1, 2, 4):
-        res = sx.dilate_interval_set(s, r, r)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            res = sx.dilate_interval_set(s, r, r)
-        torch.cuda.synchronize()
-        per = (time.perf_counter() - t0) / reps
-        byts = 8 * (s.interval_count + res.interval_count) + 8 * (s.row_count + res.row_count)
-        out[f"dilate_r{r}"] = {"ms": per * 1e3, "out_intervals": res.interval_count, "out_rows": res.row_count,
-                               "interval_ops_per_s": s.interval_count / per, "algorithmic_bytes": byts,
-                               "gbs": byts / per / 1e9}
+        for name, fn in (("dilate", lambda: sx.dilate_interval_set(s, r, r)),
+                         ("erode", lambda: sx.erode_interval_set(s, r, r, width=65536, height=height))):
+            res = fn()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                res = fn()
+            torch.cuda.synchronize()
+            per = (time.perf_counter() - t0) / reps
+            byts = 8 * (s.interval_count + res.interval_count) + 8 * (s.row_count + res.row_count)
+            out[f"{name}_r{r}"] = {"ms": per * 1e3, "out_intervals": res.interval_count, "out_rows": res.row_count,
+                                   "interval_ops_per_s": s.interval_count / per, "algorithmic_bytes": byts,
+                                   "gbs": byts / per / 1e9}
     return out
 
 

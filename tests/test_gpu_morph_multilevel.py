@@ -425,32 +425,7 @@ def test_dilate_tiles_edge_cases_vs_oracle(sx, hy):
     oracle, through dilate_interval_set and the tiled kernel directly."""
     from subsetix_amd.morphology import _dilate_tiled
 
-    rng = np.random.default_rng(90 + hy)
-    cases = []
-    # one long row among ordinary ones
-    rows = np.arange(0, 300)
-    per = [[(b, b + 2) for b in range(0, 24000, 4)] if r == 150 else
-           [(int(b), int(b) + 3) for b in np.sort(rng.choice(np.arange(0, 4000, 5), 30, replace=False))]
-           for r in rows]
-    cases.append(("long row", rows, per))
-    # unit intervals at offsets that differ from row to row: unions of
-    # shifted rows grow by the row count
-    rows = np.arange(-50, 450)
-    per = [[(16 * k + 2 * (r % 8), 16 * k + 2 * (r % 8) + 1) for k in range(70)] for r in rows]
-    cases.append(("growing unions", rows, per))
-    # coordinates spanning 2^30
-    rows = np.arange(0, 200)
-    per = [[(-(1 << 29) + 7 * r, -(1 << 29) + 7 * r + 5), (3 * r, 3 * r + 2), ((1 << 29) - 20 - r, (1 << 29) - r)]
-           for r in rows]
-    cases.append(("wide x", rows, per))
-    # row ids next to either end of the int32 range
-    top = (1 << 31) - 1 - hy
-    for name, rows in (("int32 low rows", np.arange(-(1 << 31) + hy, -(1 << 31) + hy + 40)),
-                       ("int32 high rows", np.arange(top - 60, top + 1))):
-        per = [[(int(b), int(b) + 2) for b in np.sort(rng.choice(np.arange(0, 600, 3), 12, replace=False))]
-               for _ in rows]
-        cases.append((name, rows, per))
-    for name, rows, per in cases:
+    for name, rows, per, _, _ in _tile_edge_cases(hy):
         s = _rows_set(rows, per)
         for hx in (0, 2):
             print("tiles", name, "hy", hy, "hx", hx, flush=True)
@@ -458,3 +433,82 @@ def test_dilate_tiles_edge_cases_vs_oracle(sx, hy):
             d = to_device(s)
             assert_set_equal(to_host(sx.dilate_interval_set(d, hx, hy)), exp)
             assert_set_equal(to_host(_dilate_tiled(d, hx, hy, int(s.rows[0]), int(s.rows[-1]))), exp)
+
+
+@pytest.mark.parametrize("hy", [0, 1, 2, 4])
+def test_erode_tiles_edge_cases_vs_oracle(sx, hy):
+    """The one-launch tiled erosion on sets with holes: rows of long runs
+    with a few holes each, one row with 6000 holes (its complement exceeds
+    an LDS buffer), holes at row-dependent offsets (the dilated complement
+    outgrows a buffer), a domain that cuts through the set (rows and x
+    outside [0, W) x [0, H) present), a 2^28-wide domain (keys would not
+    fit: every tile erodes row by row); against the oracle."""
+    for name, rows, per, W, H in _erode_edge_cases(hy):
+        s = _rows_set(rows, per)
+        for hx in (0, 2):
+            print("erode tiles", name, "hy", hy, "hx", hx, flush=True)
+            got = to_host(sx.erode_interval_set(to_device(s), hx, hy, width=W, height=H))
+            assert_set_equal(got, ox.erode(s, hx, hy, width=W, height=H))
+
+
+def _holes(lo, hi, holes):
+    """[lo, hi) minus unit holes at the given sorted positions."""
+    out, x = [], lo
+    for h in holes:
+        if h > x:
+            out.append((int(x), int(h)))
+        x = max(x, h + 1)
+    if hi > x:
+        out.append((int(x), int(hi)))
+    return out
+
+
+def _erode_edge_cases(hy):
+    """(name, rows, per-row intervals, width, height) for the erosion."""
+    rng = np.random.default_rng(70 + hy)
+    cases = []
+    rows = np.arange(0, 300)
+    per = [_holes(0, 20000, np.sort(rng.choice(np.arange(1, 19999, 3), 6000 if r == 150 else 30, replace=False)))
+           for r in rows]
+    cases.append(("6000 holes", rows, per, 20000, 300))
+    rows = np.arange(-40, 460)
+    per = [_holes(-50, 1200, [16 * k + 2 * (r % 8) for k in range(70)]) for r in rows]
+    cases.append(("row-dependent holes", rows, per, 1100, 400))
+    rows = np.arange(-30, 250)
+    per = [_holes(-100, 900, np.sort(rng.choice(np.arange(0, 900), 12, replace=False))) for _ in rows]
+    cases.append(("domain cut", rows, per, 700, 200))
+    rows = np.arange(0, 120)
+    per = [_holes(-(1 << 20), (1 << 27) + 9 * r, np.sort(rng.choice(np.arange(0, 1 << 26, 4096), 8, replace=False)))
+           for r in rows]
+    cases.append(("wide domain", rows, per, 1 << 28, 120))
+    return cases
+
+
+def _tile_edge_cases(hy):
+    """(name, rows, per-row intervals, erosion width, erosion height)."""
+    rng = np.random.default_rng(90 + hy)
+    cases = []
+    # one long row among ordinary ones
+    rows = np.arange(0, 300)
+    per = [[(b, b + 2) for b in range(0, 24000, 4)] if r == 150 else
+           [(int(b), int(b) + 3) for b in np.sort(rng.choice(np.arange(0, 4000, 5), 30, replace=False))]
+           for r in rows]
+    cases.append(("long row", rows, per, 20000, 260))
+    # unit intervals at offsets that differ from row to row: unions of
+    # shifted rows grow by the row count
+    rows = np.arange(-50, 450)
+    per = [[(16 * k + 2 * (r % 8), 16 * k + 2 * (r % 8) + 1) for k in range(70)] for r in rows]
+    cases.append(("growing unions", rows, per, 1000, 400))
+    # coordinates spanning 2^30
+    rows = np.arange(0, 200)
+    per = [[(-(1 << 29) + 7 * r, -(1 << 29) + 7 * r + 5), (3 * r, 3 * r + 2), ((1 << 29) - 20 - r, (1 << 29) - r)]
+           for r in rows]
+    cases.append(("wide x", rows, per, 1 << 28, 150))
+    # row ids next to either end of the int32 range
+    top = (1 << 31) - 1 - hy
+    for name, rows in (("int32 low rows", np.arange(-(1 << 31) + hy, -(1 << 31) + hy + 40)),
+                       ("int32 high rows", np.arange(top - 60, top + 1))):
+        per = [[(int(b), int(b) + 2) for b in np.sort(rng.choice(np.arange(0, 600, 3), 12, replace=False))]
+               for _ in rows]
+        cases.append((name, rows, per, 500, 60))
+    return cases
