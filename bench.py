@@ -6,7 +6,10 @@ lengths/gaps with means 40/60), batched: B independent seeded pairs stacked
 row-wise and processed by ONE fused kernel launch per operation.  A step =
 union + intersection + difference over the batch, inputs resident in HBM, no
 host synchronisation inside the step.  value = input intervals processed per
-second (the reference's ns-per-input-interval convention, inverted).
+second (the reference's ns-per-input-interval convention, inverted), from the
+wall time of the K timed steps; a second pass launches each operation K times
+back to back between two HIP events for the per-kernel times (kernel_ms, the
+roofline).
 
 Also reported beside the headline (each in its own object):
   copy           1 GiB device copy: the achievable-bandwidth reference and the

@@ -66,19 +66,26 @@ def setalgebra_batched(batch: int, steps: int, warmup: int, dev, group=None) -> 
         for op in range(3):
             run_op(op)
     torch.cuda.synchronize()
-    ev = {op: [] for op in range(3)}
+    # throughput: the steps alone (a timing event costs ~8 us of GPU time,
+    # about 10 % of a step with events around every launch)
     t0 = time.perf_counter()
     for _ in range(steps):
         for op in range(3):
-            s, e = _events()
-            s.record()
             run_op(op)
-            e.record()
-            ev[op].append((s, e))
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    # kernel times: per operation, `steps` launches back to back between two
+    # HIP events on the launch stream (the average includes the launch gaps)
+    kms = {}
+    for op in range(3):
+        s, e = _events()
+        s.record()
+        for _ in range(steps):
+            run_op(op)
+        e.record()
+        torch.cuda.synchronize()
+        kms[op] = s.elapsed_time(e) / steps
     _core.read_counts(run_op(0))
-    kms = {op: float(np.mean([s.elapsed_time(e) for s, e in ev[op]])) for op in range(3)}
     byts = {op: setop_bytes(na, nb, totals[op], R, R, R) for op in range(3)}
     return {
         "pairs": batch, "rows": R, "input_intervals_per_op": na + nb,
