@@ -481,6 +481,9 @@ def _erode_edge_cases(hy):
     per = [_holes(-(1 << 20), (1 << 27) + 9 * r, np.sort(rng.choice(np.arange(0, 1 << 26, 4096), 8, replace=False)))
            for r in rows]
     cases.append(("wide domain", rows, per, 1 << 28, 120))
+    cases.append(("empty set", np.zeros(0, dtype=np.int64), [], 50, 10))
+    rows = np.arange(-100, -90)
+    cases.append(("rows above the domain", rows, [[(0, 40)] for _ in rows], 50, 10))
     return cases
 
 
