@@ -72,8 +72,12 @@ int64_t stx_scan_scratch_bytes(int64_t n_bound);
  * _align_interval_sets, expressions.py:58-141) and an optional transform
  * (STX_XF_*).  Output: out_off[U+1], out_beg/out_end (capacity entries),
  * *d_total.  U = *d_nrows when non-NULL, else nrows_bound.
- * group <= 0: LDS tile path when no maps are given: 0 and -6: flat tiles
- * (a tile's rows merged as one keyed sequence, single pass), 1024 threads x
+ * group <= 0: LDS tile path when no maps are given: 0 and -7: flat tiles in
+ * the register-output form for two operands read as stored (else as -6);
+ * -8: the same with count-balanced tiles (a tile table built on the device
+ * from the offsets; for inputs whose density varies between row ranges,
+ * where tiles of equal row counts would overflow); -6: flat tiles (a
+ * tile's rows merged as one keyed sequence, single pass), 1024 threads x
  * 8 positions; -5: flat tiles, 512 x 8; -1: row-aware tiles, 512 x 8, single
  * pass; -2: the same in two passes; -3: row-aware 128 x 16; -4: row-aware
  * 64 x 32.  With maps, the per-row path.  4 / 16 / 64: lane-group size of
@@ -85,6 +89,9 @@ int32_t stx_setop(int32_t op, const int32_t* a_beg, const int32_t* a_end, const 
                   int32_t* out_end, int64_t capacity, int32_t* d_total, uint32_t* status, void* scratch,
                   int64_t scratch_bytes, uint32_t epoch, hipStream_t stream);
 int64_t stx_setop_scratch_bytes(int64_t nrows_bound, int32_t group);
+/* Scratch for a call with `elems` input intervals (a_n + b_n): schedule -8
+ * keeps its tile table in the scratch as well. */
+int64_t stx_setop_scratch_bytes_n(int64_t nrows_bound, int64_t elems, int32_t group);
 
 /* Allocation-free set operation.  No counterpart in the reference: it is for
  * callers that cannot allocate device memory (code under HIP graph capture,

@@ -134,7 +134,7 @@ def setop_raw(op: int, a, b, *, nrows: int, d_nrows: Optional[torch.Tensor] = No
     b_beg, b_end, b_off = b if b is not None else (None, None, None)
 
     def launch(g: int) -> None:
-        sp, sbytes, ep = ctx.scratch(_lib.lib().stx_setop_scratch_bytes(nrows, g))
+        sp, sbytes, ep = ctx.scratch(_lib.lib().stx_setop_scratch_bytes_n(nrows, na + nb, g))
         check(_lib.lib().stx_setop(int(op), ptr(a_beg), ptr(a_end), ptr(a_off), ptr(map_a), int(xf_a[0]),
                                    int(xf_a[1]), int(na), ptr(b_beg), ptr(b_end), ptr(b_off), ptr(map_b),
                                    int(xf_b[0]), int(xf_b[1]), int(nb), ptr(d_nrows), int(nrows), int(g),
@@ -145,7 +145,7 @@ def setop_raw(op: int, a, b, *, nrows: int, d_nrows: Optional[torch.Tensor] = No
     if g == 0:
         tile_ok = map_a is None and map_b is None and d_nrows is None
         plain2 = b is not None and xf_a[0] == XF_NONE and xf_b[0] == XF_NONE
-        cands = (((-7,) if plain2 else ()) + ((-6, -5, -1, -2, -3, -4) if b is not None else (-6, -5, -1, -3, -4))
+        cands = (((-7, -8) if plain2 else ()) + ((-6, -5, -1, -2, -3, -4) if b is not None else (-6, -5, -1, -3, -4))
                  if tile_ok else ()) + autotune.GROUPS
         key = ("setop", int(op), tile_ok, autotune.bucket(na + nb), autotune.bucket((na + nb) / max(nrows, 1)))
         g = autotune.pick(key, lambda: {c: (lambda c=c: launch(c)) for c in cands}) or 0

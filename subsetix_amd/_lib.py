@@ -32,6 +32,7 @@ _SIGNATURES: Dict[str, Tuple[object, list]] = {
     "stx_scan_i32": (_I32, [_P, _P, _I64, _P, _P, _P, _P, _I64, _I32, _P]),
     "stx_scan_scratch_bytes": (_I64, [_I64]),
     "stx_setop_scratch_bytes": (_I64, [_I64, _I32]),
+    "stx_setop_scratch_bytes_n": (_I64, [_I64, _I64, _I32]),
     "stx_setop": (_I32, [_I32, _P, _P, _P, _P, _I32, _I32, _I64, _P, _P, _P, _P, _I32, _I32, _I64, _P, _I64, _I32,
                          _P, _P, _P, _I64, _P, _P, _P, _I64, _I32, _P]),
     "stx_rows_union": (_I32, [_P, _I64, _I32, _P, _I64, _I32, _P, _P, _P, _P, _P, _P, _P, _I64, _I32, _P]),

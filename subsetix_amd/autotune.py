@@ -12,7 +12,9 @@ similar queries.
 
 Families and candidates:
   setop      flat tiles 1024 x 8, register-output form (-7, two operands
-             read as stored), flat tiles 1024 x 8 (-6) and 512 x 8 (-5),
+             read as stored), the same with count-balanced tiles from a
+             tile table (-8: inputs of uneven density), flat tiles
+             1024 x 8 (-6) and 512 x 8 (-5),
              tile kernel 512 x 8 single pass (-1) or two passes (-2),
              128 x 16 (-3), 64 x 32 (-4) (threads x positions per thread),
              one lane group of 4 / 16 / 64 lanes per row (4 / 16 / 64);

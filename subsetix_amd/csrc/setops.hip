@@ -723,12 +723,13 @@ struct FlatTile {
     bool flat;
 };
 
-// Stages tile `tile` and walks it, leaving its outputs compacted in s_k (a
-// tile processed one wavefront per row leaves its rows' exclusive output
-// offsets there instead), and publishes the tile's output count as its
-// look-back aggregate.  Called by every thread of the block.
+// Stages tile `tile` (rows [row0, row0 + nr), nr <= flat_rows_per_tile) and
+// walks it, leaving its outputs compacted in s_k (a tile processed one
+// wavefront per row leaves its rows' exclusive output offsets there
+// instead), and publishes the tile's output count as its look-back
+// aggregate.  Called by every thread of the block.
 template <int OP, bool XF, int TB, int VT>
-__device__ __forceinline__ FlatTile flat_produce(const Operand& A, const Operand& B, int64_t U, int32_t RPT, uint32_t tile,
+__device__ __forceinline__ FlatTile flat_produce(const Operand& A, const Operand& B, int64_t row0, int nr, uint32_t tile,
                                                  int2* __restrict__ s_k, FlatShared<TB>& sh, LookbackScratch lb,
                                                  uint32_t epoch) {
     constexpr int kTW = TB / STX_WAVE;
@@ -740,8 +741,6 @@ __device__ __forceinline__ FlatTile flat_produce(const Operand& A, const Operand
     // max-scan ignores them), or above (difference: B merges by end)
     constexpr int32_t kHiY = OP == 2 ? INT32_MAX : INT32_MIN;
     const int tid = threadIdx.x, lane = lane_id(), wave = wave_id();
-    const int64_t row0 = (int64_t)tile * RPT;
-    const int nr = (int)(U - row0 < RPT ? (U - row0 > 0 ? U - row0 : 0) : RPT);
     const int32_t a_base = A.off[row0];
     const int32_t b_base = B.off ? B.off[row0] : 0;
     const int32_t nA = A.off[row0 + nr] - a_base;
@@ -1028,7 +1027,9 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(tile_waves_p
     __shared__ uint32_t s_excl;
     LookbackScratch lb = lookback_scratch(scratch, ntiles);
     const uint32_t tile = blockIdx.x;
-    const FlatTile ft = flat_produce<OP, XF, TB, VT>(A, B, U, RPT, tile, s_k, sh, lb, epoch);
+    const int64_t row0 = (int64_t)tile * RPT;
+    const int nr = (int)(U - row0 < RPT ? (U - row0 > 0 ? U - row0 : 0) : RPT);
+    const FlatTile ft = flat_produce<OP, XF, TB, VT>(A, B, row0, nr, tile, s_k, sh, lb, epoch);
     if (wave_id() == 0) {
         const uint32_t e = lookback_exclusive<1, true>(lb, tile, (uint32_t)ft.total, epoch, status, spin_limit);
         if (lane_id() == 0) s_excl = e;
@@ -1066,12 +1067,12 @@ constexpr int kFlat2Base = 8;  // first A slot; A's lower sentinel sits at slot 
 // Runs one tile through setop_flat_kernel's tile code (the flat layout of
 // that kernel, or one wavefront per row), look-back and store included.
 template <int OP, int TB, int VT>
-__device__ __forceinline__ void flat_tile_whole(const Operand& A, const Operand& B, int64_t U, int32_t RPT, uint32_t tile,
-                                                int2* s_k, FlatShared<TB>& sh, LookbackScratch lb, uint32_t epoch,
-                                                int32_t* out_off, int32_t* ob, int32_t* oe, int64_t capacity,
-                                                int32_t* d_total, uint32_t* status, int64_t spin_limit,
-                                                uint32_t* s_excl) {
-    const FlatTile ft = flat_produce<OP, false, TB, VT>(A, B, U, RPT, tile, s_k, sh, lb, epoch);
+__device__ __forceinline__ void flat_tile_whole(const Operand& A, const Operand& B, int64_t U, int64_t row0, int nr,
+                                                uint32_t tile, int2* s_k, FlatShared<TB>& sh, LookbackScratch lb,
+                                                uint32_t epoch, int32_t* out_off, int32_t* ob, int32_t* oe,
+                                                int64_t capacity, int32_t* d_total, uint32_t* status,
+                                                int64_t spin_limit, uint32_t* s_excl) {
+    const FlatTile ft = flat_produce<OP, false, TB, VT>(A, B, row0, nr, tile, s_k, sh, lb, epoch);
     if (wave_id() == 0) {
         const uint32_t e = lookback_exclusive<1, true>(lb, tile, (uint32_t)ft.total, epoch, status, spin_limit);
         if (lane_id() == 0) *s_excl = e;
@@ -1082,9 +1083,9 @@ __device__ __forceinline__ void flat_tile_whole(const Operand& A, const Operand&
 
 template <int OP, int TB, int VT>
 __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(tile_waves_per_eu<VT>()))) void setop_flat2_kernel(
-    Operand A, Operand B, int64_t U, int32_t RPT, uint32_t ntiles, int32_t* __restrict__ out_off,
-    int32_t* __restrict__ ob, int32_t* __restrict__ oe, int64_t capacity, int32_t* __restrict__ d_total,
-    uint32_t* status, void* scratch, uint32_t epoch, int64_t spin_limit) {
+    Operand A, Operand B, int64_t U, int32_t RPT, const int32_t* __restrict__ tab, uint32_t ntiles,
+    int32_t* __restrict__ out_off, int32_t* __restrict__ ob, int32_t* __restrict__ oe, int64_t capacity,
+    int32_t* __restrict__ d_total, uint32_t* status, void* scratch, uint32_t epoch, int64_t spin_limit) {
     static_assert(VT == 8, "one 8-interval chunk per thread");
     constexpr int kTW = TB / STX_WAVE;
     static_assert(kTW <= 16, "the range and scan combines read one 16-lane row");
@@ -1103,8 +1104,20 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(tile_waves_p
     LookbackScratch lb = lookback_scratch(scratch, ntiles);
     const uint32_t tile = blockIdx.x;
     const int tid = threadIdx.x, lane = lane_id(), wave = wave_id();
-    const int64_t row0 = (int64_t)tile * RPT;
-    const int nr = (int)(U - row0 < RPT ? (U - row0 > 0 ? U - row0 : 0) : RPT);
+    // rows of the tile: RPT per tile, or the tile table's count-balanced
+    // starts (tiles past the last row start at U, and no tile waits on them)
+    int64_t row0;
+    int nr;
+    if (tab) {
+        row0 = tab[tile];
+        if (row0 >= U && tile > 0) return;
+        const int64_t n = (int64_t)tab[tile + 1] - row0;
+        nr = (int)(n < kMaxRows ? n : kMaxRows);
+        if (n > kMaxRows && tid == 0) atomicOr(status, STX_FLAG_CAPACITY);  // a table not made for these counts
+    } else {
+        row0 = (int64_t)tile * RPT;
+        nr = (int)(U - row0 < RPT ? (U - row0 > 0 ? U - row0 : 0) : RPT);
+    }
     const int32_t a_base = A.off[row0];
     const int32_t b_base = B.off[row0];
     const int32_t nA = A.off[row0 + nr] - a_base;
@@ -1113,8 +1126,8 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(tile_waves_p
     constexpr int32_t a0 = kFlat2Base;
     const int32_t b0 = (a0 + nA + VT + 2 + 7) & ~7;  // past A's VT + 1 upper sentinels and B's lower one
     if (!(b0 + nB + VT + 1 <= kPCAP && nr <= kMaxRows)) {  // uniform: the tile does not fit this layout
-        flat_tile_whole<OP, TB, VT>(A, B, U, RPT, tile, s_k, *reinterpret_cast<FlatShared<TB>*>(s_aux), lb, epoch,
-                                    out_off, ob, oe, capacity, d_total, status, spin_limit, &s_excl);
+        flat_tile_whole<OP, TB, VT>(A, B, U, row0, nr, tile, s_k, *reinterpret_cast<FlatShared<TB>*>(s_aux), lb,
+                                    epoch, out_off, ob, oe, capacity, d_total, status, spin_limit, &s_excl);
         return;
     }
     int32_t* s_ra = s_aux;
@@ -1181,8 +1194,8 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(tile_waves_p
     const uint32_t shift = range > 0 ? 64u - (uint32_t)__clzll((unsigned long long)range) : 0u;
     if (((int64_t)nr << shift) >= (1ll << 31)) {  // uniform: rows x key band do not fit 31 bits
         __syncthreads();  // every thread has read s_red / s_aux before the tile code reuses s_aux
-        flat_tile_whole<OP, TB, VT>(A, B, U, RPT, tile, s_k, *reinterpret_cast<FlatShared<TB>*>(s_aux), lb, epoch,
-                                    out_off, ob, oe, capacity, d_total, status, spin_limit, &s_excl);
+        flat_tile_whole<OP, TB, VT>(A, B, U, row0, nr, tile, s_k, *reinterpret_cast<FlatShared<TB>*>(s_aux), lb,
+                                    epoch, out_off, ob, oe, capacity, d_total, status, spin_limit, &s_excl);
         return;
     }
     const int32_t xmin = lo;
@@ -1592,16 +1605,50 @@ static void launch_flat(int32_t op, uint32_t ntiles, hipStream_t stream, const O
 template <int TB, int VT>
 static void launch_flat2(int32_t op, uint32_t ntiles, hipStream_t stream, const Operand& A, const Operand& B, int64_t U,
                          int32_t rpt, int32_t* out_off, int32_t* ob, int32_t* oe, int64_t capacity, int32_t* d_total,
-                         uint32_t* status, void* scratch, uint32_t epoch, int64_t spin) {
+                         uint32_t* status, void* scratch, uint32_t epoch, int64_t spin,
+                         const int32_t* tab = nullptr) {
     if (op == 0)
-        hipLaunchKernelGGL((setop_flat2_kernel<0, TB, VT>), dim3(ntiles), dim3(TB), 0, stream, A, B, U, rpt, ntiles,
-                           out_off, ob, oe, capacity, d_total, status, scratch, epoch, spin);
+        hipLaunchKernelGGL((setop_flat2_kernel<0, TB, VT>), dim3(ntiles), dim3(TB), 0, stream, A, B, U, rpt, tab,
+                           ntiles, out_off, ob, oe, capacity, d_total, status, scratch, epoch, spin);
     else if (op == 1)
-        hipLaunchKernelGGL((setop_flat2_kernel<1, TB, VT>), dim3(ntiles), dim3(TB), 0, stream, A, B, U, rpt, ntiles,
-                           out_off, ob, oe, capacity, d_total, status, scratch, epoch, spin);
+        hipLaunchKernelGGL((setop_flat2_kernel<1, TB, VT>), dim3(ntiles), dim3(TB), 0, stream, A, B, U, rpt, tab,
+                           ntiles, out_off, ob, oe, capacity, d_total, status, scratch, epoch, spin);
     else
-        hipLaunchKernelGGL((setop_flat2_kernel<2, TB, VT>), dim3(ntiles), dim3(TB), 0, stream, A, B, U, rpt, ntiles,
-                           out_off, ob, oe, capacity, d_total, status, scratch, epoch, spin);
+        hipLaunchKernelGGL((setop_flat2_kernel<2, TB, VT>), dim3(ntiles), dim3(TB), 0, stream, A, B, U, rpt, tab,
+                           ntiles, out_off, ob, oe, capacity, d_total, status, scratch, epoch, spin);
+}
+
+// Count-balanced tiles (schedule -8): tile k starts at the first row r with
+// h(r) >= k, h(r) = c(r) / T + r / M, c(r) the intervals of both operands
+// in rows below r.  Two starts in a row differ in h by one, so a tile holds
+// fewer than T intervals plus its last row, and at most M rows.  tab[k] for
+// k = 0 .. ntiles; starts past h(U) are U.  One thread per row (writing the
+// starts that fall on it) and per table entry past h(U); *status gets
+// STX_FLAG_CAPACITY when the operands hold more intervals than the tile
+// count was planned for.
+constexpr int64_t kBalancedRoom = 1024 * 8 - 34;  // intervals a 1024 x 8 register-output tile holds
+constexpr int64_t kBalancedTarget = kBalancedRoom - 256;  // room left for the tile's last row
+
+__global__ void tile_table_kernel(const int32_t* __restrict__ a_off, const int32_t* __restrict__ b_off, int64_t U,
+                                  int64_t T, int64_t M, int64_t ntiles, int32_t* __restrict__ tab,
+                                  uint32_t* status) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t a0 = a_off[0], b0 = b_off[0];
+    auto h = [&](int64_t r) { return ((int64_t)a_off[r] - a0 + (int64_t)b_off[r] - b0) / T + r / M; };
+    const int64_t hU = h(U);
+    if (i <= U) {
+        const int64_t hi = h(i), lo = i > 0 ? h(i - 1) : -1;
+        for (int64_t k = lo + 1; k <= hi && k <= ntiles; ++k) tab[k] = (int32_t)i;
+    }
+    if (i > hU && i <= ntiles) tab[i] = (int32_t)U;
+    if (i == 0 && hU >= ntiles) atomicOr(status, STX_FLAG_CAPACITY);
+}
+
+inline int64_t balanced_tiles(int64_t elems, int64_t U) { return elems / kBalancedTarget + U / 1024 + 1; }
+
+inline int64_t balanced_scratch_bytes(int64_t elems, int64_t U) {
+    const int64_t nt = balanced_tiles(elems, U);
+    return (lookback_bytes(nt) + 15) / 16 * 16 + 4 * (nt + 1);
 }
 
 template <int G>
@@ -1694,6 +1741,13 @@ int64_t stx_setop_scratch_bytes(int64_t nrows_bound, int32_t group) {
                : lookback_bytes(nrows_bound + 1);
 }
 
+int64_t stx_setop_scratch_bytes_n(int64_t nrows_bound, int64_t elems, int32_t group) {
+    const int64_t base = stx_setop_scratch_bytes(nrows_bound, group);
+    if (group != -8) return base;
+    const int64_t b = balanced_scratch_bytes(elems > 0 ? elems : 0, nrows_bound);
+    return b > base ? b : base;
+}
+
 int32_t stx_setop(int32_t op, const int32_t* a_beg, const int32_t* a_end, const int32_t* a_off, const int32_t* a_map,
                   int32_t a_xf_mode, int32_t a_xf_param, int64_t a_n, const int32_t* b_beg, const int32_t* b_end,
                   const int32_t* b_off, const int32_t* b_map, int32_t b_xf_mode, int32_t b_xf_param, int64_t b_n,
@@ -1720,6 +1774,21 @@ int32_t stx_setop(int32_t op, const int32_t* a_beg, const int32_t* a_end, const 
             return stx_launch_status("setop_flat_kernel");
         }
         const bool plain2 = a_xf_mode == XF_NONE && b_off != nullptr && b_xf_mode == XF_NONE;
+        if (group == -8 && plain2 && nrows_bound < INT32_MAX) {
+            // register-output tiles with count-balanced rows: a tile table
+            // in the scratch after the look-back states
+            const int64_t nt = balanced_tiles(elems, nrows_bound);
+            if (balanced_scratch_bytes(elems, nrows_bound) > scratch_bytes)
+                return stx_fail(STX_EINVAL, "setop: scratch too small (stx_setop_scratch_bytes_n)");
+            int32_t* tab = (int32_t*)((char*)scratch + (lookback_bytes(nt) + 15) / 16 * 16);
+            const int64_t nthreads = (nrows_bound > nt ? nrows_bound : nt) + 1;
+            hipLaunchKernelGGL(tile_table_kernel, dim3((unsigned)((nthreads + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                               stream, a_off, b_off, nrows_bound, kBalancedTarget, (int64_t)flat_rows_per_tile<1024>(),
+                               nt, tab, status);
+            launch_flat2<1024, 8>(op, (uint32_t)nt, stream, A, B, nrows_bound, 1, out_off, out_beg, out_end, capacity,
+                                  d_total, status, scratch, epoch, spin, tab);
+            return stx_launch_status("setop_flat2_kernel(balanced)");
+        }
         if ((group == -7 || (group == 0 && kFlatDefault)) && plain2) {
             // flat tiles in the register-output form, 1024 x 8 (two per CU),
             // filled to 95 %; other operands take the flat tiles below
@@ -1730,7 +1799,7 @@ int32_t stx_setop(int32_t op, const int32_t* a_beg, const int32_t* a_end, const 
                                   d_total, status, scratch, epoch, spin);
             return stx_launch_status("setop_flat2_kernel");
         }
-        if (group == -6 || group == -7 || (group == 0 && kFlatDefault)) {
+        if (group == -6 || group == -7 || group == -8 || (group == 0 && kFlatDefault)) {
             // flat tiles, 1024 x 8 (two per CU), filled to 95 %: the mean
             // tile holds ~100 rows or more, so its size varies little
             const int32_t rpt = setop_tile_rows<1024, 8>(elems, nrows_bound, kFlatFill, flat_rows_per_tile<1024>());

@@ -61,7 +61,7 @@ def install():
     autotune._DEFAULT = old
 
 
-@pytest.mark.parametrize("k", range(10))
+@pytest.mark.parametrize("k", range(11))
 def test_setop_candidates_vs_oracle(torch_mod, install, k):
     from subsetix_amd.expressions import binary_op
 
@@ -84,7 +84,7 @@ def test_setop_candidates_vs_oracle(torch_mod, install, k):
     assert "setop" in fams
     tile_keys = [names for key, names in t.ran.items() if key[0] == "setop" and key[2]]
     map_keys = [names for key, names in t.ran.items() if key[0] == "setop" and not key[2]]
-    assert tile_keys and all(n == [-7, -6, -5, -1, -2, -3, -4, 4, 16, 64] for n in tile_keys)
+    assert tile_keys and all(n == [-7, -8, -6, -5, -1, -2, -3, -4, 4, 16, 64] for n in tile_keys)
     assert map_keys and all(n == [4, 16, 64] for n in map_keys)
 
 

@@ -55,7 +55,7 @@ def test_golden_cell_model(sx, golden):
             assert ox.cells(to_host(got)) == cell_model(spec, cells), spec
 
 
-@pytest.mark.parametrize("group", [None, -2, -3, -4, -5, -6, -7, 4, 16, 64])  # None / negative: LDS tile shapes
+@pytest.mark.parametrize("group", [None, -2, -3, -4, -5, -6, -7, -8, 4, 16, 64])  # None / negative: LDS tile shapes
 @pytest.mark.parametrize("seed", [0, 1, 2])
 def test_random_vs_oracle(sx, seed, group):
     from subsetix_amd import _core
@@ -197,15 +197,15 @@ def test_row_count_multiple_of_tile_rows(sx):
             assert_set_equal(to_host(_ops(sx)[op](da, db)), ox.setop(op, a, b))
 
 
-@pytest.mark.parametrize("schedule", [-1, -2, -3, -4, -5, -6, -7])
+@pytest.mark.parametrize("schedule", [-1, -2, -3, -4, -5, -6, -7, -8])
 def test_tile_schedules_batched_pairs(sx, schedule):
     """Tile schedules on 16 stacked full-size pairs (hundreds of tiles),
     bit-exact against the oracle: single pass 512 x 8 (-1, decoupled
     look-back), two passes (-2, count kernel, prefix kernel, write kernel),
     single pass 128 x 16 (-3) and 64 x 32 (-4), flat tiles 512 x 8 (-5) and
-    1024 x 8 (-6), flat tiles in the register-output form (-7); repeated launches on one scratch buffer alternate
-    schedules.  Dilation (transforms on load, one operand) runs through the same shape (-7 takes the -6 tiles
-    there)."""
+    1024 x 8 (-6), flat tiles in the register-output form (-7) and the same with count-balanced tiles (-8);
+    repeated launches on one scratch buffer alternate schedules.  Dilation (transforms on load, one operand) runs
+    through the same shape (-7 and -8 take the -6 tiles there)."""
     import torch
 
     from subsetix_amd import _core, workloads
@@ -296,7 +296,7 @@ def _flat_edge_pair(seed):
     return _rows_to_oset(A), _rows_to_oset(B)
 
 
-@pytest.mark.parametrize("schedule", [-1, -5, -6, -7])
+@pytest.mark.parametrize("schedule", [-1, -5, -6, -7, -8])
 @pytest.mark.parametrize("seed", [0, 1])
 def test_flat_tile_edge_cases(sx, schedule, seed):
     """Every branch of the flat tile kernel (and the same inputs through the
@@ -330,6 +330,65 @@ def test_flat_tile_edge_cases(sx, schedule, seed):
         assert_set_equal(to_host(sx.clip_interval_set(ds, width=60, height=R)), ox.clip(small, 60, R))
     finally:
         _core.set_group_override(None)
+
+
+def _side_by_side(s, t, shift):
+    """Rows of s followed, in the same row, by the rows of t moved right by
+    shift (twice the intervals per row)."""
+    off = s.off.astype(np.int64) + t.off.astype(np.int64)
+    beg = np.empty(int(off[-1]), np.int32)
+    end = np.empty(int(off[-1]), np.int32)
+    for r in range(s.off.size - 1):
+        na = s.off[r + 1] - s.off[r]
+        o = off[r]
+        beg[o:o + na], end[o:o + na] = s.beg[s.off[r]:s.off[r + 1]], s.end[s.off[r]:s.off[r + 1]]
+        beg[o + na:off[r + 1]] = t.beg[t.off[r]:t.off[r + 1]] + shift
+        end[o + na:off[r + 1]] = t.end[t.off[r]:t.off[r + 1]] + shift
+    return ox.make(off, beg, end, s.rows)
+
+
+@pytest.mark.parametrize("schedule", [None, -7, -8])
+def test_uneven_density_stack_vs_oracle(sx, schedule):
+    """Stacked instances whose density alternates (every other instance has
+    twice the intervals per row), where tiles of equal row counts planned at
+    the mean row length overflow in the dense instances: the count-balanced
+    tiles (-8), the equal-row tiles (-7) and the default, bit-exact against
+    the oracle for the three operations."""
+    from subsetix_amd import _core
+
+    rng = np.random.default_rng(11)
+    A, B = [], []
+    for k in range(10):
+        a = random_canonical(rng, 600, 60000, mean_n=30)
+        b = random_canonical(rng, 600, 60000, mean_n=30)
+        if k % 2:
+            a = _side_by_side(a, random_canonical(rng, 600, 60000, mean_n=30), 65536)
+            b = _side_by_side(b, random_canonical(rng, 600, 60000, mean_n=30), 65536)
+        A.append(a)
+        B.append(b)
+
+    def stack(sets):
+        offs, begs, ends, base = [], [], [], 0
+        for x in sets:
+            offs.append(x.off[:-1].astype(np.int64) + base)
+            begs.append(x.beg)
+            ends.append(x.end)
+            base += x.beg.size
+        off = np.concatenate(offs + [np.array([base])])
+        return ox.make(off, np.concatenate(begs), np.concatenate(ends), np.arange(off.size - 1, dtype=np.int32))
+
+    a, b = stack(A), stack(B)
+    da, db = to_device(a), to_device(b)
+    for op, name in ((0, "union"), (1, "intersection"), (2, "difference")):
+        print("uneven density", schedule, name, flush=True)
+        exp = ox.setop(name, a, b)
+        out_off, ob, oe, ctx = _core.setop_raw(op, (da.begin, da.end, da.row_offsets),
+                                               (db.begin, db.end, db.row_offsets), nrows=da.row_count,
+                                               group=schedule)
+        (total,) = _core.read_counts(ctx, ctx.words[1])
+        np.testing.assert_array_equal(out_off.cpu().numpy(), exp.off)
+        np.testing.assert_array_equal(ob[:total].cpu().numpy(), exp.beg)
+        np.testing.assert_array_equal(oe[:total].cpu().numpy(), exp.end)
 
 
 @pytest.mark.parametrize("seed", [3, 4, 5])
