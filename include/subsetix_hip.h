@@ -182,7 +182,8 @@ int32_t stx_dilate_max_halo(void);
  * (row_lo / row_hi: the set's first and last row id; nt = row_hi - row_lo +
  * 1 + 2 hy): tiles of consecutive output rows each merge their 2 hy + 1
  * widened input rows in LDS, then the rows without intervals are dropped.
- * Outputs as stx_dilate (row_cap >= nt, capacity >= (2 hy + 1) N).  tmp:
+ * Outputs as stx_dilate (row_cap >= nt, capacity >= (2 hy + 1) N); row ids
+ * other than the set's first / last set STX_FLAG_ROWS.  tmp:
  * stx_dilate_dense_tmp_ints(nt, hy) int32; scratch:
  * stx_dilate_dense_scratch_bytes(nt) with epochs epoch and epoch + 1. */
 /* Erosion (erode_interval_set, morphology.py:361-386: U - clip(dilate(U -

@@ -807,6 +807,13 @@ __global__ __launch_bounds__(kDTB) __attribute__((amdgpu_waves_per_eu(8))) void 
     }
 }
 
+// The caller's first and last row id against the set's (STX_FLAG_ROWS if
+// they differ: rows outside the dense range would be dropped silently).
+__global__ void row_bounds_check_kernel(const int32_t* __restrict__ rows, int64_t R, int32_t lo, int32_t hi,
+                                        uint32_t* status) {
+    if (threadIdx.x == 0 && R > 0 && (rows[0] != lo || rows[R - 1] != hi)) atomicOr(status, STX_FLAG_ROWS);
+}
+
 __global__ void dense_rows_kernel(int32_t* __restrict__ rows, int64_t n, int32_t y0) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) rows[i] = (int32_t)((int64_t)y0 + i);
@@ -899,6 +906,7 @@ int32_t stx_dilate_dense(const int32_t* rows, const int32_t* off, int64_t R, con
     int32_t* off_d = D + (nt + 2 * (int64_t)hy + 1);
     int32_t* rows_d = off_d + (nt + 1);
     int32_t* ctmp = rows_d + nt;
+    hipLaunchKernelGGL(row_bounds_check_kernel, dim3(1), dim3(64), 0, stream, rows, R, row_lo, row_hi, status);
     int32_t rc = stx_dense_offsets(rows, off, R, (int64_t)row_lo - 2 * hy, nt + 2 * hy, D, stream);
     if (rc) return rc;
     // staged input per tile ~ kDFill of a buffer at the mean row length, so

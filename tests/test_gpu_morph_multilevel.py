@@ -402,6 +402,10 @@ def test_dilate_windows_vs_oracle(sx, hy):
         if hy <= _lib.lib().stx_dilate_max_halo():
             assert_set_equal(to_host(_dilate_fused(to_device(s2), hx, hy)), exp)
             assert_set_equal(to_host(_dilate_tiled(to_device(s2), hx, hy, lo, hi)), exp)
+    if hy <= _lib.lib().stx_dilate_max_halo():
+        # a row range that is not the set's is reported, not silently cropped
+        with pytest.raises(_lib.StxError, match="row ids out of range"):
+            _dilate_tiled(to_device(s2), 1, hy, lo, hi - 1)
 
 
 def _rows_set(rows, per_row):
