@@ -697,6 +697,10 @@ __device__ __forceinline__ void flat_load(const Operand& A, const Operand& B, in
     }
 }
 
+// Look-back polls of the flat tiles that find nothing ready sleep
+// kFlatSleep x 64 cycles (see lookback_exclusive).
+constexpr int kFlatSleep = 64;
+
 // Staged-key slot i lives at i + i / 16 in LDS: one spare slot per 16
 // spreads the strided accesses of the staging and of the merge walks (lanes
 // 4 to 8 slots apart) over all banks.
@@ -1031,7 +1035,8 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(tile_waves_p
     const int nr = (int)(U - row0 < RPT ? (U - row0 > 0 ? U - row0 : 0) : RPT);
     const FlatTile ft = flat_produce<OP, XF, TB, VT>(A, B, row0, nr, tile, s_k, sh, lb, epoch);
     if (wave_id() == 0) {
-        const uint32_t e = lookback_exclusive<1, true>(lb, tile, (uint32_t)ft.total, epoch, status, spin_limit);
+        const uint32_t e =
+            lookback_exclusive<1, true, kFlatSleep>(lb, tile, (uint32_t)ft.total, epoch, status, spin_limit);
         if (lane_id() == 0) s_excl = e;
     }
     __syncthreads();
@@ -1074,7 +1079,8 @@ __device__ __forceinline__ void flat_tile_whole(const Operand& A, const Operand&
                                                 int64_t spin_limit, uint32_t* s_excl) {
     const FlatTile ft = flat_produce<OP, false, TB, VT>(A, B, row0, nr, tile, s_k, sh, lb, epoch);
     if (wave_id() == 0) {
-        const uint32_t e = lookback_exclusive<1, true>(lb, tile, (uint32_t)ft.total, epoch, status, spin_limit);
+        const uint32_t e =
+            lookback_exclusive<1, true, kFlatSleep>(lb, tile, (uint32_t)ft.total, epoch, status, spin_limit);
         if (lane_id() == 0) *s_excl = e;
     }
     __syncthreads();
@@ -1345,7 +1351,8 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(tile_waves_p
             s_out[2 * tile_total - 1] = (int32_t)(((uint32_t)imax(carry, lm) & kmask) + (uint32_t)xmin);
     }
     if (wave == 0) {
-        const uint32_t e = lookback_exclusive<1, true>(lb, tile, (uint32_t)tile_total, epoch, status, spin_limit);
+        const uint32_t e =
+            lookback_exclusive<1, true, kFlatSleep>(lb, tile, (uint32_t)tile_total, epoch, status, spin_limit);
         if (lane == 0) s_excl = e;
     }
     __syncthreads();
