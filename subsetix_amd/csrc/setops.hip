@@ -697,9 +697,9 @@ __device__ __forceinline__ void flat_load(const Operand& A, const Operand& B, in
     }
 }
 
-// Look-back polls of the flat tiles that find nothing ready sleep
-// kFlatSleep x 64 cycles (see lookback_exclusive).
-constexpr int kFlatSleep = 64;
+// Look-back of the flat tiles: peek at the nearest predecessor, sleeping
+// kFlatSleep x 64 cycles while it has not published (see lookback_exclusive).
+constexpr int kFlatSleep = 16;
 
 // Staged-key slot i lives at i + i / 16 in LDS: one spare slot per 16
 // spreads the strided accesses of the staging and of the merge walks (lanes
@@ -1036,7 +1036,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(tile_waves_p
     const FlatTile ft = flat_produce<OP, XF, TB, VT>(A, B, row0, nr, tile, s_k, sh, lb, epoch);
     if (wave_id() == 0) {
         const uint32_t e =
-            lookback_exclusive<1, true, kFlatSleep>(lb, tile, (uint32_t)ft.total, epoch, status, spin_limit);
+            lookback_exclusive<1, true, kFlatSleep, true>(lb, tile, (uint32_t)ft.total, epoch, status, spin_limit);
         if (lane_id() == 0) s_excl = e;
     }
     __syncthreads();
@@ -1080,7 +1080,7 @@ __device__ __forceinline__ void flat_tile_whole(const Operand& A, const Operand&
     const FlatTile ft = flat_produce<OP, false, TB, VT>(A, B, row0, nr, tile, s_k, sh, lb, epoch);
     if (wave_id() == 0) {
         const uint32_t e =
-            lookback_exclusive<1, true, kFlatSleep>(lb, tile, (uint32_t)ft.total, epoch, status, spin_limit);
+            lookback_exclusive<1, true, kFlatSleep, true>(lb, tile, (uint32_t)ft.total, epoch, status, spin_limit);
         if (lane_id() == 0) *s_excl = e;
     }
     __syncthreads();
@@ -1352,7 +1352,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(tile_waves_p
     }
     if (wave == 0) {
         const uint32_t e =
-            lookback_exclusive<1, true, kFlatSleep>(lb, tile, (uint32_t)tile_total, epoch, status, spin_limit);
+            lookback_exclusive<1, true, kFlatSleep, true>(lb, tile, (uint32_t)tile_total, epoch, status, spin_limit);
         if (lane == 0) s_excl = e;
     }
     __syncthreads();

@@ -171,11 +171,14 @@ __device__ __forceinline__ void lookback_publish(LookbackScratch s, uint32_t til
 // poll starts behind them.  After `spin_limit` polls that found no progress
 // the block records STX_FLAG_SPIN in `status` and proceeds (its result is
 // then invalid, but every wave still exits).  A poll that finds nothing
-// ready sleeps SLEEP x 64 cycles first: long-lived tiles poll less and leave
-// the memory system to the tiles still loading (64 for the 1024-thread set
-// tiles: union 159 -> 153 us on 128 stacked pairs; 96 and 127 were slower
-// again, as was a doubling backoff).
-template <int K = 1, bool PUBLISHED = false, int SLEEP = 1>
+// ready sleeps SLEEP x 64 cycles first, and with PEEK a wave first reads only
+// the nearest unfolded predecessor's word (one 8-byte load) until that one
+// has published: long-lived tiles then poll less and leave the memory system
+// to the tiles still loading (the 1024-thread set tiles on 128 stacked
+// pairs: union 159 -> 153 us with a 64 x 64-cycle sleep, 150 us with the
+// peek and a 16 x 64-cycle sleep; sleeps of 96 or 127 were slower, as was a
+// doubling backoff).
+template <int K = 1, bool PUBLISHED = false, int SLEEP = 1, bool PEEK = false>
 __device__ __forceinline__ uint32_t lookback_exclusive(LookbackScratch s, uint32_t tile, uint32_t agg, uint32_t epoch,
                                                        uint32_t* status, int64_t spin_limit) {
     const int lane = lane_id();
@@ -188,6 +191,18 @@ __device__ __forceinline__ uint32_t lookback_exclusive(LookbackScratch s, uint32
     int64_t pos = (int64_t)tile - 1;  // nearest predecessor not yet folded in
     int64_t spins = 0;
     while (pos >= 0) {
+        if (PEEK) {
+            const unsigned long long w0 = state_load(&s.state[pos]);
+            const uint32_t hi0 = (uint32_t)(w0 >> 32);
+            if ((hi0 >> 2) != epoch || (hi0 & 3u) == 0u) {  // not published yet
+                if (++spins > spin_limit) {
+                    if (lane == 0) atomicOr(status, STX_FLAG_SPIN);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(SLEEP);
+                continue;
+            }
+        }
         unsigned long long w[K];
 #pragma unroll
         for (int k = 0; k < K; ++k) {
