@@ -15,8 +15,9 @@ Also reported beside the headline (each in its own object):
   copy           1 GiB device copy: the achievable-bandwidth reference and the
                  calibration kernel for the HBM byte counters
   plain_form     one pair through the reference-shaped evaluate() call
-  morphology     dilate / erode, square SE radius 1, 2, 4 (one pair), and dilation of
-                 64 instances stacked with 16-row gaps in one call
+  morphology     dilate / erode, square SE radius 1, 2, 4 (one pair), and dilation /
+                 erosion of 64 instances stacked with 16-row gaps in one call
+                 (and of 32 instances alternating 16 / 64 intervals per row)
   large_stencil  8192^2 fine / 4096^2 coarse composite, 500 steps, FP32 vs FP64
   batched_amr    512 three-level instances, 200 steps, regrid every 8, device
                  regrid, each regrid + 8 steps one hipGraph replay; exclusive

@@ -180,29 +180,31 @@ int32_t stx_fill_i32(int32_t* p, int64_t n, int32_t value, int32_t step, hipStre
 int32_t stx_dilate_max_halo(void);
 /* The same dilation over the dense output rows row_lo - hy .. row_hi + hy
  * (row_lo / row_hi: the set's first and last row id; nt = row_hi - row_lo +
- * 1 + 2 hy): tiles of consecutive output rows each merge their 2 hy + 1
- * widened input rows in LDS, then the rows without intervals are dropped.
+ * 1 + 2 hy): tiles of consecutive output rows, balanced by interval count
+ * (a table in the scratch), each merge their 2 hy + 1 widened input rows in
+ * LDS, then the rows without intervals are dropped.
  * Outputs as stx_dilate (row_cap >= nt, capacity >= (2 hy + 1) N); row ids
  * other than the set's first / last set STX_FLAG_ROWS.  tmp:
  * stx_dilate_dense_tmp_ints(nt, hy) int32; scratch:
- * stx_dilate_dense_scratch_bytes(nt) with epochs epoch and epoch + 1. */
+ * stx_dilate_dense_scratch_bytes(nt, N) with epochs epoch and epoch + 1. */
 /* Erosion (erode_interval_set, morphology.py:361-386: U - clip(dilate(U -
  * clip(S))), U = [0, W) x [0, H), the exterior counting as inside) in one
- * launch of row tiles, for 0 <= hy <= stx_dilate_max_halo(): each tile takes
+ * launch of row tiles balanced by interval count (a table in the scratch),
+ * for 0 <= hy <= stx_dilate_max_halo(): each tile takes
  * the complement inside [0, W) of its rows' clipped input window (rows
  * outside [0, H) give none), its gaps widened by hx, the 2 hy + 1 row union
  * and the complement of that.  Outputs: off_out (H + 1 entries; the rows are
  * 0 .. H - 1), beg_out / end_out (capacity >= (2 hy + 1)(N + H) + H),
  * *d_total.  tmp: stx_erode_dense_tmp_ints(H, hy) int32; scratch:
- * stx_erode_dense_scratch_bytes(H), one epoch. */
+ * stx_erode_dense_scratch_bytes(H, N), one epoch. */
 int64_t stx_erode_dense_tmp_ints(int64_t H, int32_t hy);
-int64_t stx_erode_dense_scratch_bytes(int64_t H);
+int64_t stx_erode_dense_scratch_bytes(int64_t H, int64_t N);
 int32_t stx_erode_dense(const int32_t* rows, const int32_t* off, int64_t R, const int32_t* beg, const int32_t* end,
                         int64_t N, int32_t hx, int32_t hy, int32_t W, int64_t H, int32_t* off_out, int32_t* beg_out,
                         int32_t* end_out, int64_t capacity, int32_t* d_total, int32_t* tmp, int64_t tmp_ints,
                         uint32_t* status, void* scratch, int64_t scratch_bytes, uint32_t epoch, hipStream_t stream);
 int64_t stx_dilate_dense_tmp_ints(int64_t nt, int32_t hy);
-int64_t stx_dilate_dense_scratch_bytes(int64_t nt);
+int64_t stx_dilate_dense_scratch_bytes(int64_t nt, int64_t N);
 int32_t stx_dilate_dense(const int32_t* rows, const int32_t* off, int64_t R, const int32_t* beg, const int32_t* end,
                          int64_t N, int32_t hx, int32_t hy, int32_t row_lo, int32_t row_hi, int32_t* rows_out,
                          int32_t* off_out, int32_t* beg_out, int32_t* end_out, int64_t row_cap, int64_t capacity,

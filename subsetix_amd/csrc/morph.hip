@@ -362,9 +362,9 @@ __device__ __forceinline__ int32_t window_union_steps(int2 (*buf)[dsk_size(kDCap
 template <int KMAX>
 __global__ __launch_bounds__(kDTB) __attribute__((amdgpu_waves_per_eu(8))) void dilate_tile_kernel(
     const int32_t* __restrict__ D, const int32_t* __restrict__ beg, const int32_t* __restrict__ end, int64_t nt,
-    int32_t hx, int32_t hy, int32_t rpt, uint32_t ntiles, int32_t* __restrict__ out_off, int32_t* __restrict__ ob,
-    int32_t* __restrict__ oe, int64_t capacity, int32_t* __restrict__ d_total, uint32_t* status, void* scratch,
-    uint32_t epoch, int64_t spin_limit) {
+    int32_t hx, int32_t hy, const int2* __restrict__ tab, uint32_t ntiles, int32_t* __restrict__ out_off,
+    int32_t* __restrict__ ob, int32_t* __restrict__ oe, int64_t capacity, int32_t* __restrict__ d_total,
+    uint32_t* status, void* scratch, uint32_t epoch, int64_t spin_limit) {
     constexpr int kTW = kDTB / STX_WAVE;
     constexpr int kPerThread = kDCap / kDTB;
     __shared__ __align__(16) int2 s_buf[2][dsk_size(kDCap)];
@@ -377,12 +377,18 @@ __global__ __launch_bounds__(kDTB) __attribute__((amdgpu_waves_per_eu(8))) void 
     const uint32_t tile = blockIdx.x;
     const int tid = threadIdx.x, lane = lane_id(), wave = wave_id();
     const int K = 2 * hy + 1;
-    const int64_t i0 = (int64_t)tile * rpt;
-    const int nr = (int)(nt - i0 < rpt ? nt - i0 : rpt);  // ntiles = ceil(nt / rpt): nr >= 1
-    const int Rt = nr + 2 * hy;                            // staged input rows [i0, i0 + Rt)
+    // output rows [i0, i0 + nr) from the count-balanced table (tiles past
+    // the last row start at nt, and no tile waits on them; an empty tile
+    // before a row too large for one tile publishes 0)
+    const int64_t i0 = tab[tile].x;
+    if (i0 >= nt && tile > 0) return;
+    const int64_t n_rows = (int64_t)tab[tile + 1].x - i0;
+    if (n_rows > kDTB && tid == 0) atomicOr(status, STX_FLAG_CAPACITY);  // a table not made for these rows
+    const int nr = (int)(n_rows < kDTB ? n_rows : kDTB);
+    const int Rt = nr + 2 * hy;  // staged input rows [i0, i0 + Rt)
     const int32_t base = D[i0];
     const int32_t nIn = D[i0 + Rt] - base;
-    bool lds_ok = nIn <= kDCap;  // uniform
+    bool lds_ok = nr > 0 && nIn <= kDCap;  // uniform
     for (int r = tid; r <= Rt; r += kDTB) s_rs[r] = D[i0 + r] - base;
     // this thread's intervals: kPerThread consecutive ones from q0
     const int32_t q0 = tid * kPerThread;
@@ -697,7 +703,7 @@ __device__ int32_t erode_row(const int32_t* __restrict__ D, const int32_t* __res
 template <int KMAX>
 __global__ __launch_bounds__(kDTB) __attribute__((amdgpu_waves_per_eu(8))) void erode_tile_kernel(
     const int32_t* __restrict__ D, const int32_t* __restrict__ beg, const int32_t* __restrict__ end, int64_t H,
-    int32_t W, int32_t hx, int32_t hy, uint32_t shift, int32_t rpt, uint32_t ntiles, int32_t* __restrict__ out_off,
+    int32_t W, int32_t hx, int32_t hy, uint32_t shift, const int2* __restrict__ tab, uint32_t ntiles, int32_t* __restrict__ out_off,
     int32_t* __restrict__ ob, int32_t* __restrict__ oe, int64_t capacity, int32_t* __restrict__ d_total,
     uint32_t* status, void* scratch, uint32_t epoch, int64_t spin_limit) {
     constexpr int kTW = kDTB / STX_WAVE;
@@ -712,12 +718,16 @@ __global__ __launch_bounds__(kDTB) __attribute__((amdgpu_waves_per_eu(8))) void 
     const uint32_t tile = blockIdx.x;
     const int tid = threadIdx.x, lane = lane_id(), wave = wave_id();
     const int K = 2 * hy + 1;
-    const int64_t i0 = (int64_t)tile * rpt;
-    const int nr = (int)(H - i0 < rpt ? H - i0 : rpt);  // ntiles = ceil(H / rpt): nr >= 1
-    const int Rt = nr + 2 * hy;                           // staged rows i0 - hy .. i0 + nr + hy - 1
+    // rows [i0, i0 + nr) from the count-balanced table, as for the dilation
+    const int64_t i0 = tab[tile].x;
+    if (i0 >= H && tile > 0) return;
+    const int64_t n_rows = (int64_t)tab[tile + 1].x - i0;
+    if (n_rows > kDTB && tid == 0) atomicOr(status, STX_FLAG_CAPACITY);
+    const int nr = (int)(n_rows < kDTB ? n_rows : kDTB);
+    const int Rt = nr + 2 * hy;  // staged rows i0 - hy .. i0 + nr + hy - 1
     const int32_t base = D[i0];
     const int32_t nIn = D[i0 + Rt] - base;
-    bool lds_ok = nIn <= kDCap && ((int64_t)Rt << shift) < (1ll << 30);  // uniform
+    bool lds_ok = nr > 0 && nIn <= kDCap && ((int64_t)Rt << shift) < (1ll << 30);  // uniform
     for (int r = tid; r <= Rt; r += kDTB) s_rs[r] = D[i0 + r] - base;
     const int32_t q0 = tid * kPerThread;
     const int32_t n_mine = lds_ok ? (nIn - q0 < 0 ? 0 : (nIn - q0 > kPerThread ? kPerThread : nIn - q0)) : 0;
@@ -821,6 +831,50 @@ __global__ void dense_rows_kernel(int32_t* __restrict__ rows, int64_t n, int32_t
 
 }  // namespace
 
+// Count-balanced row tiles over dense offsets D (D row j holds row j - hy
+// of the tile rows' numbering, so tile row i is D row i + hy): with c(i) =
+// the intervals of D rows hy .. hy + i - 1 plus per_row for each of those
+// rows (the erosion's complement adds up to one gap per row) and M = kDTB,
+// h(i) = floor(c(i) / T + i / M), tile k starts at the first row with h >=
+// k.  Between a tile's first and last row h grows by less than one, so its
+// rows hold fewer than T intervals plus its last row, and there are at most
+// M of them; unlike floor(c / T) + floor(i / M) no tile is cut every M rows
+// (that cut 8 % more tiles on uniform stacks).  A tile's staged window adds
+// hy rows on each side.  Table entries as in balanced_table_entry.
+__global__ void dense_tile_table_kernel(const int32_t* __restrict__ D, int64_t U, int32_t hy, int32_t per_row,
+                                        int64_t T, int64_t ntiles, int2* __restrict__ tab, uint32_t* status) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t c0 = D[hy];
+    auto h = [&](int64_t r) { return (((int64_t)D[r + hy] - c0 + (int64_t)per_row * r) * kDTB + r * T) / (T * kDTB); };
+    balanced_table_entry(i, U, ntiles, h, tab, status);
+}
+
+// Table target for tiles staged to `fill` of a buffer: the tile's own rows
+// get t = fill x kDCap less `halo` x the expected 2 hy halo rows of `mean`
+// intervals, at least a quarter buffer (which bounds the tile count for the
+// scratch size), raised by 1 / (1 - t / (M mean)) for the rows' own share of
+// h, so that tiles of rows at the mean length hold t intervals.
+inline int64_t dense_tile_target(double fill, double halo, int32_t hy, double mean) {
+    double t = fill * kDCap - halo * 2.0 * hy * mean;
+    if (t < kDCap / 4) t = kDCap / 4;
+    const double rows_share = t / (kDTB * (mean > 1e-3 ? mean : 1e-3));
+    return (int64_t)(rows_share < 0.5 ? t / (1.0 - rows_share) : 2.0 * t);
+}
+
+// Tiles of the table: h(U) + 1 for rows of total weight `weight`.
+inline int64_t dense_tiles(int64_t weight, int64_t T, int64_t U) { return (weight * kDTB + U * T) / (T * kDTB) + 1; }
+
+// Writes the table of `ntiles` tiles over U rows after the look-back states
+// in the scratch; returns it.
+static int2* dense_tile_table(const int32_t* D, int64_t U, int32_t hy, int32_t per_row, int64_t T, int64_t ntiles,
+                              void* scratch, uint32_t* status, hipStream_t stream) {
+    int2* tab = (int2*)((char*)scratch + balanced_table_offset(ntiles));
+    const int64_t n = (U > ntiles ? U : ntiles) + 1;
+    hipLaunchKernelGGL(dense_tile_table_kernel, dim3(nblk(n)), dim3(kT), 0, stream, D, U, hy, per_row, T, ntiles, tab,
+                       status);
+    return tab;
+}
+
 extern "C" int64_t stx_scan_scratch_bytes(int64_t n_bound);
 extern "C" int32_t stx_dense_offsets(const int32_t* rows, const int32_t* off, int64_t R, int64_t y0, int64_t n,
                                      int32_t* out, hipStream_t stream);
@@ -883,8 +937,9 @@ int64_t stx_dilate_dense_tmp_ints(int64_t nt, int32_t hy) {
     return (nt + 2 * (int64_t)hy + 1) + (nt + 1) + nt + 2 * (nt + 1);
 }
 
-int64_t stx_dilate_dense_scratch_bytes(int64_t nt) {
-    const int64_t lb = lookback_bytes(nt), sc = stx_scan_scratch_bytes(nt);
+int64_t stx_dilate_dense_scratch_bytes(int64_t nt, int64_t N) {
+    const int64_t lb = balanced_table_scratch_bytes(dense_tiles(N > 0 ? N : 0, kDCap / 4, nt));
+    const int64_t sc = stx_scan_scratch_bytes(nt);
     return lb > sc ? lb : sc;
 }
 
@@ -901,7 +956,8 @@ int32_t stx_dilate_dense(const int32_t* rows, const int32_t* off, int64_t R, con
     const int64_t nt = span + 2 * (int64_t)hy;
     if (row_cap < nt) return stx_fail(STX_EINVAL, "dilate_dense: row capacity below the dense row count");
     if (tmp_ints < stx_dilate_dense_tmp_ints(nt, hy)) return stx_fail(STX_EINVAL, "dilate_dense: tmp too small");
-    if (scratch_bytes < stx_dilate_dense_scratch_bytes(nt)) return stx_fail(STX_EINVAL, "dilate_dense: scratch too small");
+    if (scratch_bytes < stx_dilate_dense_scratch_bytes(nt, N))
+        return stx_fail(STX_EINVAL, "dilate_dense: scratch too small");
     int32_t* D = tmp;
     int32_t* off_d = D + (nt + 2 * (int64_t)hy + 1);
     int32_t* rows_d = off_d + (nt + 1);
@@ -909,30 +965,27 @@ int32_t stx_dilate_dense(const int32_t* rows, const int32_t* off, int64_t R, con
     hipLaunchKernelGGL(row_bounds_check_kernel, dim3(1), dim3(64), 0, stream, rows, R, row_lo, row_hi, status);
     int32_t rc = stx_dense_offsets(rows, off, R, (int64_t)row_lo - 2 * hy, nt + 2 * hy, D, stream);
     if (rc) return rc;
-    // staged input per tile ~ kDFill of a buffer at the mean row length, so
-    // that the steps (the first may grow the count) rarely overflow: on the
-    // 64-instance stack 0.7 gave 0.23 / 0.27 / 0.29 ms at r = 1 / 2 / 4, 0.8
-    // 0.55 / 0.40 / 0.28 ms (tiles falling back to row merges), 0.6 0.24 /
-    // 0.27 / 0.31 ms
+    // count-balanced tiles staging about kDFill of a buffer, so that the
+    // steps (the first may grow the count) rarely overflow: with tiles of
+    // equal rows planned at the mean row length, the 64-instance stack took
+    // 0.23 / 0.27 / 0.29 ms at r = 1 / 2 / 4 with 0.7, 0.55 / 0.40 / 0.28 ms
+    // with 0.8 (tiles falling back to row merges), 0.24 / 0.27 / 0.31 ms
+    // with 0.6
     constexpr double kDFill = 0.7;
-    const double mean = (double)N / (double)span;
-    int64_t rpt = (int64_t)(kDFill * kDCap / (mean > 1.0 ? mean : 1.0)) - 2 * hy;
-    if (rpt < 1) rpt = 1;
-    if (rpt > kDTB) rpt = kDTB;
-    const int64_t ntiles = (nt + rpt - 1) / rpt;
+    const int64_t T = dense_tile_target(kDFill, 1.0, hy, (double)N / (double)span);
+    const int64_t ntiles = dense_tiles(N, T, nt);
+    if (balanced_table_scratch_bytes(ntiles) > scratch_bytes)
+        return stx_fail(STX_EINVAL, "dilate_dense: scratch too small");
+    const int2* tab = dense_tile_table(D, nt, hy, 0, T, ntiles, scratch, status, stream);
     const int K = 2 * hy + 1;
-    if (K <= 3)
-        hipLaunchKernelGGL((dilate_tile_kernel<3>), dim3((unsigned)ntiles), dim3(kDTB), 0, stream, D, beg, end, nt, hx, hy,
-                           (int32_t)rpt, (uint32_t)ntiles, off_d, beg_out, end_out, capacity, d_total, status, scratch,
-                           epoch, stx_spin_limit());
-    else if (K <= 5)
-        hipLaunchKernelGGL((dilate_tile_kernel<5>), dim3((unsigned)ntiles), dim3(kDTB), 0, stream, D, beg, end, nt, hx, hy,
-                           (int32_t)rpt, (uint32_t)ntiles, off_d, beg_out, end_out, capacity, d_total, status, scratch,
-                           epoch, stx_spin_limit());
-    else
-        hipLaunchKernelGGL((dilate_tile_kernel<9>), dim3((unsigned)ntiles), dim3(kDTB), 0, stream, D, beg, end, nt, hx, hy,
-                           (int32_t)rpt, (uint32_t)ntiles, off_d, beg_out, end_out, capacity, d_total, status, scratch,
-                           epoch, stx_spin_limit());
+#define STX_DILATE_LAUNCH(KM)                                                                                          \
+    hipLaunchKernelGGL((dilate_tile_kernel<KM>), dim3((unsigned)ntiles), dim3(kDTB), 0, stream, D, beg, end, nt, hx, hy, \
+                       tab, (uint32_t)ntiles, off_d, beg_out, end_out, capacity, d_total, status, scratch, epoch,       \
+                       stx_spin_limit())
+    if (K <= 3) STX_DILATE_LAUNCH(3);
+    else if (K <= 5) STX_DILATE_LAUNCH(5);
+    else STX_DILATE_LAUNCH(9);
+#undef STX_DILATE_LAUNCH
     rc = stx_launch_status("dilate_tile_kernel");
     if (rc) return rc;
     hipLaunchKernelGGL(dense_rows_kernel, dim3(nblk(nt)), dim3(kT), 0, stream, rows_d, nt, (int32_t)(row_lo - hy));
@@ -942,7 +995,9 @@ int32_t stx_dilate_dense(const int32_t* rows, const int32_t* off, int64_t R, con
 
 int64_t stx_erode_dense_tmp_ints(int64_t H, int32_t hy) { return H + 2 * (int64_t)hy + 1; }
 
-int64_t stx_erode_dense_scratch_bytes(int64_t H) { return lookback_bytes(H > 1 ? H : 1); }
+int64_t stx_erode_dense_scratch_bytes(int64_t H, int64_t N) {
+    return balanced_table_scratch_bytes(dense_tiles((N > 0 ? N : 0) + (H > 0 ? H : 0), kDCap / 4, H > 1 ? H : 1));
+}
 
 int32_t stx_erode_dense(const int32_t* rows, const int32_t* off, int64_t R, const int32_t* beg, const int32_t* end,
                         int64_t N, int32_t hx, int32_t hy, int32_t W, int64_t H, int32_t* off_out, int32_t* beg_out,
@@ -951,7 +1006,8 @@ int32_t stx_erode_dense(const int32_t* rows, const int32_t* off, int64_t R, cons
     if (R < 0 || N < 0 || hx < 0 || hy < 0 || hy > kMaxHalo || W < 1 || H < 1 || H > INT32_MAX)
         return stx_fail(STX_EINVAL, "erode_dense: need R, N, hx >= 0, 0 <= hy <= 4, W >= 1, 1 <= H < 2^31");
     if (tmp_ints < stx_erode_dense_tmp_ints(H, hy)) return stx_fail(STX_EINVAL, "erode_dense: tmp too small");
-    if (scratch_bytes < stx_erode_dense_scratch_bytes(H)) return stx_fail(STX_EINVAL, "erode_dense: scratch too small");
+    if (scratch_bytes < stx_erode_dense_scratch_bytes(H, N))
+        return stx_fail(STX_EINVAL, "erode_dense: scratch too small");
     int32_t* D = tmp;  // row j of D holds row j - hy, j = 0 .. H + 2 hy - 1
     int32_t rc = stx_dense_offsets(rows, off, R, -(int64_t)hy, H + 2 * hy, D, stream);
     if (rc) return rc;
@@ -961,17 +1017,23 @@ int32_t stx_erode_dense(const int32_t* rows, const int32_t* off, int64_t R, cons
     // the complement holds about one interval more per row than the set;
     // its widened gaps only merge from there, so the staged count is the
     // tile's largest (0.9 of a buffer: 0.22 / 0.24 / 0.25 ms against 0.25 /
-    // 0.27 / 0.31 ms at 0.7 on the 64-instance stack, r = 1 / 2 / 4)
+    // 0.27 / 0.31 ms at 0.7 on the 64-instance stack, r = 1 / 2 / 4, tiles
+    // of equal rows); count-balanced tiles, a row weighing its intervals + 1,
+    // the halo allowance 1.5 x the mean halo: at this fill the longer halo
+    // rows of denser regions matter (with 1 x, the tiles of the dense
+    // instances of a stack alternating 16 / 64 intervals per row overflowed
+    // at r = 4: 3.3 ms against 0.21 ms; uniform stacks lose about 5 % at r =
+    // 4 to the smaller tiles)
     constexpr double kDFill = 0.9;
-    const double mean = (double)N / (double)H + 1.0;
-    int64_t rpt = (int64_t)(kDFill * kDCap / mean) - 2 * hy;
-    if (rpt < 1) rpt = 1;
-    if (rpt > kDTB) rpt = kDTB;
-    const int64_t ntiles = (H + rpt - 1) / rpt;
+    const int64_t T = dense_tile_target(kDFill, 1.5, hy, (double)N / (double)H + 1.0);
+    const int64_t ntiles = dense_tiles(N + H, T, H);
+    if (balanced_table_scratch_bytes(ntiles) > scratch_bytes)
+        return stx_fail(STX_EINVAL, "erode_dense: scratch too small");
+    const int2* tab = dense_tile_table(D, H, hy, 1, T, ntiles, scratch, status, stream);
     const int K = 2 * hy + 1;
 #define STX_ERODE_LAUNCH(KM)                                                                                         \
     hipLaunchKernelGGL((erode_tile_kernel<KM>), dim3((unsigned)ntiles), dim3(kDTB), 0, stream, D, beg, end, H, W, hx, \
-                       hy, shift, (int32_t)rpt, (uint32_t)ntiles, off_out, beg_out, end_out, capacity, d_total, status, \
+                       hy, shift, tab, (uint32_t)ntiles, off_out, beg_out, end_out, capacity, d_total, status, \
                        scratch, epoch, stx_spin_limit())
     if (K <= 1) STX_ERODE_LAUNCH(1);
     else if (K <= 3) STX_ERODE_LAUNCH(3);

@@ -1089,7 +1089,7 @@ __device__ __forceinline__ void flat_tile_whole(const Operand& A, const Operand&
 
 template <int OP, int TB, int VT>
 __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(tile_waves_per_eu<VT>()))) void setop_flat2_kernel(
-    Operand A, Operand B, int64_t U, int32_t RPT, const int32_t* __restrict__ tab, uint32_t ntiles,
+    Operand A, Operand B, int64_t U, int32_t RPT, const int2* __restrict__ tab, uint32_t ntiles,
     int32_t* __restrict__ out_off, int32_t* __restrict__ ob, int32_t* __restrict__ oe, int64_t capacity,
     int32_t* __restrict__ d_total, uint32_t* status, void* scratch, uint32_t epoch, int64_t spin_limit) {
     static_assert(VT == 8, "one 8-interval chunk per thread");
@@ -1115,9 +1115,9 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(tile_waves_p
     int64_t row0;
     int nr;
     if (tab) {
-        row0 = tab[tile];
+        row0 = tab[tile].x;
         if (row0 >= U && tile > 0) return;
-        const int64_t n = (int64_t)tab[tile + 1] - row0;
+        const int64_t n = (int64_t)tab[tile + 1].x - row0;
         nr = (int)(n < kMaxRows ? n : kMaxRows);
         if (n > kMaxRows && tid == 0) atomicOr(status, STX_FLAG_CAPACITY);  // a table not made for these counts
     } else {
@@ -1613,7 +1613,7 @@ template <int TB, int VT>
 static void launch_flat2(int32_t op, uint32_t ntiles, hipStream_t stream, const Operand& A, const Operand& B, int64_t U,
                          int32_t rpt, int32_t* out_off, int32_t* ob, int32_t* oe, int64_t capacity, int32_t* d_total,
                          uint32_t* status, void* scratch, uint32_t epoch, int64_t spin,
-                         const int32_t* tab = nullptr) {
+                         const int2* tab = nullptr) {
     if (op == 0)
         hipLaunchKernelGGL((setop_flat2_kernel<0, TB, VT>), dim3(ntiles), dim3(TB), 0, stream, A, B, U, rpt, tab,
                            ntiles, out_off, ob, oe, capacity, d_total, status, scratch, epoch, spin);
@@ -1625,37 +1625,25 @@ static void launch_flat2(int32_t op, uint32_t ntiles, hipStream_t stream, const 
                            ntiles, out_off, ob, oe, capacity, d_total, status, scratch, epoch, spin);
 }
 
-// Count-balanced tiles (schedule -8): tile k starts at the first row r with
-// h(r) >= k, h(r) = c(r) / T + r / M, c(r) the intervals of both operands
-// in rows below r.  Two starts in a row differ in h by one, so a tile holds
-// fewer than T intervals plus its last row, and at most M rows.  tab[k] for
-// k = 0 .. ntiles; starts past h(U) are U.  One thread per row (writing the
-// starts that fall on it) and per table entry past h(U); *status gets
-// STX_FLAG_CAPACITY when the operands hold more intervals than the tile
-// count was planned for.
+// Count-balanced tiles (schedule -8): balanced_table_entry with c(r) the
+// intervals of both operands in rows below r, so a tile holds fewer than T
+// intervals plus its last row, and at most M rows.  One thread per row and
+// per table entry past h(U).
 constexpr int64_t kBalancedRoom = 1024 * 8 - 34;  // intervals a 1024 x 8 register-output tile holds
 constexpr int64_t kBalancedTarget = kBalancedRoom - 256;  // room left for the tile's last row
 
 __global__ void tile_table_kernel(const int32_t* __restrict__ a_off, const int32_t* __restrict__ b_off, int64_t U,
-                                  int64_t T, int64_t M, int64_t ntiles, int32_t* __restrict__ tab,
-                                  uint32_t* status) {
+                                  int64_t T, int64_t M, int64_t ntiles, int2* __restrict__ tab, uint32_t* status) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t a0 = a_off[0], b0 = b_off[0];
     auto h = [&](int64_t r) { return ((int64_t)a_off[r] - a0 + (int64_t)b_off[r] - b0) / T + r / M; };
-    const int64_t hU = h(U);
-    if (i <= U) {
-        const int64_t hi = h(i), lo = i > 0 ? h(i - 1) : -1;
-        for (int64_t k = lo + 1; k <= hi && k <= ntiles; ++k) tab[k] = (int32_t)i;
-    }
-    if (i > hU && i <= ntiles) tab[i] = (int32_t)U;
-    if (i == 0 && hU >= ntiles) atomicOr(status, STX_FLAG_CAPACITY);
+    balanced_table_entry(i, U, ntiles, h, tab, status);
 }
 
 inline int64_t balanced_tiles(int64_t elems, int64_t U) { return elems / kBalancedTarget + U / 1024 + 1; }
 
 inline int64_t balanced_scratch_bytes(int64_t elems, int64_t U) {
-    const int64_t nt = balanced_tiles(elems, U);
-    return (lookback_bytes(nt) + 15) / 16 * 16 + 4 * (nt + 1);
+    return balanced_table_scratch_bytes(balanced_tiles(elems, U));
 }
 
 template <int G>
@@ -1787,7 +1775,7 @@ int32_t stx_setop(int32_t op, const int32_t* a_beg, const int32_t* a_end, const 
             const int64_t nt = balanced_tiles(elems, nrows_bound);
             if (balanced_scratch_bytes(elems, nrows_bound) > scratch_bytes)
                 return stx_fail(STX_EINVAL, "setop: scratch too small (stx_setop_scratch_bytes_n)");
-            int32_t* tab = (int32_t*)((char*)scratch + (lookback_bytes(nt) + 15) / 16 * 16);
+            int2* tab = (int2*)((char*)scratch + balanced_table_offset(nt));
             const int64_t nthreads = (nrows_bound > nt ? nrows_bound : nt) + 1;
             hipLaunchKernelGGL(tile_table_kernel, dim3((unsigned)((nthreads + kBlock - 1) / kBlock)), dim3(kBlock), 0,
                                stream, a_off, b_off, nrows_bound, kBalancedTarget, (int64_t)flat_rows_per_tile<1024>(),

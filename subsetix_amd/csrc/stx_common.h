@@ -248,6 +248,36 @@ __device__ __forceinline__ uint32_t lookback_exclusive(LookbackScratch s, uint32
 // Bytes of look-back scratch needed for `ntiles` tiles.
 __host__ __device__ inline int64_t lookback_bytes(int64_t ntiles) { return 64 + 8 * ntiles; }
 
+// Count-balanced tiles over U rows.  h(r) = floor(c(r) / T) + floor(r / M)
+// with c non-decreasing, c(0) = 0, c(r) the weight of the rows below r: tile
+// k starts at the first row r with h(r) >= k.  Two starts in a row differ in
+// h by one, so a tile's rows weigh less than T plus its last row, and there
+// are at most M of them; a row weighing more than T gives empty tiles
+// before it.  Entry k of the table (k = 0 .. ntiles) is the 8-byte word
+// {start row, 0}, starts past h(U) being U: with upper half 0 a table left
+// in the look-back scratch never reads as a published state (epochs start
+// at 1), whatever later launch uses those words.  Thread i writes the starts
+// that fall on row i (i <= U) and entry i past h(U); *status gets
+// STX_FLAG_CAPACITY when h(U) >= ntiles (the table was sized for less).
+template <class HFn>
+__device__ __forceinline__ void balanced_table_entry(int64_t i, int64_t U, int64_t ntiles, HFn h, int2* tab,
+                                                     uint32_t* status) {
+    const int64_t hU = h(U);
+    if (i <= U) {
+        const int64_t hi = h(i), lo = i > 0 ? h(i - 1) : -1;
+        for (int64_t k = lo + 1; k <= hi && k <= ntiles; ++k) tab[k] = make_int2((int32_t)i, 0);
+    }
+    if (i > hU && i <= ntiles) tab[i] = make_int2((int32_t)U, 0);
+    if (i == 0 && hU >= ntiles) atomicOr(status, STX_FLAG_CAPACITY);
+}
+
+// Byte offset of a tile table placed after the look-back states of ntiles
+// tiles, and the scratch bytes of both.
+__host__ __device__ inline int64_t balanced_table_offset(int64_t ntiles) { return (lookback_bytes(ntiles) + 15) / 16 * 16; }
+__host__ __device__ inline int64_t balanced_table_scratch_bytes(int64_t ntiles) {
+    return balanced_table_offset(ntiles) + 8 * (ntiles + 1);
+}
+
 // ---- wave scans through DPP ------------------------------------------------
 // Hillis-Steele inside each 16-lane row (row_shr 1, 2, 4, 8), then
 // row_bcast:15 / row_bcast:31 carry the row totals across the wave (GFX9

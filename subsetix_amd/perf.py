@@ -161,29 +161,39 @@ def morphology_batched(batch: int, reps: int, dev) -> Dict:
     the stack's bounding domain 65536 x (last row + 1) (one launch of row
     tiles, one host read; the empty rows between instances are inside that
     domain, so each instance also erodes from its top and bottom rows).  Algorithmic bytes: 8 B per input and output interval and
-    4 B per input and output row id and offset."""
+    4 B per input and output row id and offset.  The same calls on an uneven
+    stack ("uneven": ``batch`` / 2 instances of 4096 x 65536 alternating
+    Poisson(16) and Poisson(64) intervals per row), where the row tiles'
+    count balancing matters.  Two untimed calls before each timed batch (the
+    first ones on a shape allocate device memory)."""
     import subsetix_amd as sx
+
+    def timed(s, height):
+        out = {"rows": s.row_count, "input_intervals": s.interval_count, "erode_domain": [65536, height]}
+        for r in (1, 2, 4):
+            for name, fn in (("dilate", lambda: sx.dilate_interval_set(s, r, r)),
+                             ("erode", lambda: sx.erode_interval_set(s, r, r, width=65536, height=height))):
+                res = fn()
+                res = fn()
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                for _ in range(reps):
+                    res = fn()
+                torch.cuda.synchronize()
+                per = (time.perf_counter() - t0) / reps
+                byts = 8 * (s.interval_count + res.interval_count) + 8 * (s.row_count + res.row_count)
+                out[f"{name}_r{r}"] = {"ms": per * 1e3, "out_intervals": res.interval_count,
+                                       "out_rows": res.row_count, "interval_ops_per_s": s.interval_count / per,
+                                       "algorithmic_bytes": byts, "gbs": byts / per / 1e9}
+        return out
 
     sets = [workloads.setalgebra_pair(seed_a=1 + 2 * k, seed_b=2 + 2 * k)[0] for k in range(batch)]
     h = workloads.stack_rows_gapped(sets, 16)
-    s = _dev_set(h, dev, dense=False)
-    height = int(h.rows[-1]) + 1
-    out = {"instances": batch, "rows": s.row_count, "input_intervals": s.interval_count,
-           "erode_domain": [65536, height]}
-    for r in (1, 2, 4):
-        for name, fn in (("dilate", lambda: sx.dilate_interval_set(s, r, r)),
-                         ("erode", lambda: sx.erode_interval_set(s, r, r, width=65536, height=height))):
-            res = fn()
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            for _ in range(reps):
-                res = fn()
-            torch.cuda.synchronize()
-            per = (time.perf_counter() - t0) / reps
-            byts = 8 * (s.interval_count + res.interval_count) + 8 * (s.row_count + res.row_count)
-            out[f"{name}_r{r}"] = {"ms": per * 1e3, "out_intervals": res.interval_count, "out_rows": res.row_count,
-                                   "interval_ops_per_s": s.interval_count / per, "algorithmic_bytes": byts,
-                                   "gbs": byts / per / 1e9}
+    out = {"instances": batch, **timed(_dev_set(h, dev, dense=False), int(h.rows[-1]) + 1)}
+    sets = [workloads.random_rows_set(4096, 65536, seed=200 + k, mean_count=64 if k % 2 else 16)
+            for k in range(max(batch // 2, 2))]
+    h = workloads.stack_rows_gapped(sets, 16)
+    out["uneven"] = {"instances": len(sets), **timed(_dev_set(h, dev, dense=False), int(h.rows[-1]) + 1)}
     return out
 
 

@@ -408,6 +408,25 @@ def test_dilate_windows_vs_oracle(sx, hy):
             _dilate_tiled(to_device(s2), 1, hy, lo, hi - 1)
 
 
+@pytest.mark.parametrize("r", [1, 2, 4])
+def test_morph_uneven_density_vs_oracle(sx, r):
+    """Dilation and erosion of a stack whose instances alternate between 20
+    and 80 intervals per row (600 rows each, 16 empty rows between them):
+    the row tiles are balanced by interval count, so the dense instances get
+    shorter tiles; bit-exact against the oracle on the whole stack."""
+    from subsetix_amd import workloads
+
+    sets = [workloads.random_rows_set(600, 65536, seed=30 + 8 * r + k, mean_count=80 if k % 2 else 20, mean_len=20,
+                                      mean_gap=20) for k in range(8)]
+    st = hostset_to_oset(workloads.stack_rows_gapped(sets, 16))
+    d = to_device(st)
+    print("uneven stack r", r, "intervals", st.beg.size, flush=True)
+    assert_set_equal(to_host(sx.dilate_interval_set(d, r, r)), ox.dilate(st, r, r))
+    W, H = 65536, int(st.rows[-1]) + 1
+    got = to_host(sx.erode_interval_set(d, r, r, width=W, height=H))
+    assert_set_equal(got, ox.erode(st, r, r, width=W, height=H))
+
+
 def _rows_set(rows, per_row):
     off, beg, end = [0], [], []
     for iv in per_row:
